@@ -1,0 +1,170 @@
+/*
+ * meumapps_ss.h -- C ABI of the MI355X semi-implicit spectral step of MEUMAPPS-SS.
+ *
+ * The reference has no plugin API: the time step is inline in the main
+ * program (Multi_MEUMAPPS.f:548-1106) and its only library seam is the
+ * P3DFFT module called from f_trans / inv_trans (ModFunc.f:15-40).  This
+ * header is the seam that replaces both: the Fortran driver keeps input
+ * parsing, set-up, initial conditions and dumps, hands the post-set-up
+ * ModGparams state over once (mm_params), uploads the fields once, and then
+ * calls mm_step in place of the loop body.  The whole periodic grid stays in
+ * the GPU's HBM between calls.
+ *
+ * Conventions
+ *   - Plain C types only; every host array is caller-owned and laid out the
+ *     way the reference's Fortran arrays are (column-major, component index
+ *     fastest for phi(V,Nx,Ny,Nz) and Cons(E,Nx,Ny,Nz)).
+ *   - Every function returns 0 on success or one of the MM_ERR_* codes; the
+ *     positive codes reuse the reference's err_flag values (ModFunc.f:44-72,
+ *     Manual p.9) so a driver can print the reference's STOP messages.
+ *   - One context = one device, one HIP stream; a context is not reentrant.
+ */
+#ifndef MEUMAPPS_SS_H
+#define MEUMAPPS_SS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_MAX_VAR 16 /* order parameters (vartot)                       */
+#define MM_MAX_PH 4   /* phases including the matrix (numph)             */
+#define MM_MAX_EL 4   /* independent compositions (numel after the -1)   */
+#define MM_GP_STRIDE (2 * MM_MAX_EL + 1) /* rows of Gparams per phase    */
+
+/* precision of the step */
+#define MM_FP64 0  /* every field, FFT and pointwise kernel in binary64       */
+#define MM_MIXED 1 /* FP32 FFTs and pointwise kernels; FP64 k=0 mode, mass
+                      sums and semi-implicit denominators                     */
+
+/* return codes */
+#define MM_OK 0
+#define MM_ERR_ZSOLVE 2      /* KKS Zintf singular   (err_flag 2, Multi_MEUMAPPS.f:690) */
+#define MM_ERR_QINV 3        /* acoustic tensor      (err_flag 3, ModElastic.f:187)     */
+#define MM_ERR_CINV_ITER 4   /* 6x6 compliance mix   (err_flag 4, ModElastic.f:270)     */
+#define MM_ERR_CINV_ENER 6   /* 6x6 compliance mix   (err_flag 6, ModElastic.f:599)     */
+#define MM_ERR_NEWFAC 7      /* elastic backtracking newfac < 1e-7 (ModElastic.f:446)    */
+#define MM_ERR_ARG 20001     /* bad argument / unsupported size                         */
+#define MM_ERR_NOMEM 20002   /* device allocation failed                                */
+#define MM_ERR_HIP 20003     /* HIP runtime error                                       */
+#define MM_ERR_NODEV 20004   /* no GPU visible                                          */
+
+/*
+ * Post-set-up contents of ModGparams (ModGparams.f:6-90) that the step reads.
+ * Multi-dimensional Fortran arrays are stored column-major with a fixed
+ * leading stride so the struct can be mirrored by a BIND(C) derived type:
+ *   grad_coeff(i,j,v) -> grad_coeff[(v-1)*9 + (j-1)*3 + (i-1)]  (already / del_N**2)
+ *   trans_ppt(m,v)    -> trans_ppt[(v-1)*6 + (m-1)]
+ *   CEla(i,j,p)       -> cela[(p-1)*36 + (j-1)*6 + (i-1)]        (after eltensor)
+ *   SEla(i,j,p)       -> sela[(p-1)*36 + (j-1)*6 + (i-1)]        (= CEla^-1)
+ *   Gparams(j,p)      -> gparams[(p-1)*MM_GP_STRIDE + (j-1)]     (after the /1.055e-5
+ *                        conversion and disc_ener, Multi_MEUMAPPS.f:218-236)
+ *   Nvar(p)           -> nvar[p-1]  (nvar[0] == 1: the matrix)
+ */
+typedef struct mm_params {
+  int32_t nx, ny, nz;      /* Nx, Ny, Nz (even, 2^a 3^b, 8..512)                 */
+  int32_t vartot;          /* V                                                  */
+  int32_t numel;           /* E (numel - 1 of Input.txt)                         */
+  int32_t numph;           /* phases including the matrix                       */
+  int32_t nvar[MM_MAX_PH]; /* Nvar(1..numph)                                     */
+  int32_t step;            /* current value of the reference's `step`            */
+  int32_t elastep0;        /* elasticity runs when step >= elastep0              */
+  int32_t noise_step;      /* Langevin noise is added while step <= noise_step   */
+  int32_t pad0;
+  double t_step;           /* dt                                                 */
+  double b_mob;            /* Allen-Cahn mobility L                              */
+  double cmobs[MM_MAX_EL]; /* Cahn-Hilliard mobility per composition           */
+  double grad_coeff[MM_MAX_VAR * 9];
+  double trans_ppt[MM_MAX_VAR * 6];
+  double cela[MM_MAX_PH * 36];
+  double sela[MM_MAX_PH * 36];
+  double gparams[MM_MAX_PH * MM_GP_STRIDE];
+  double wellph[MM_MAX_PH];
+  double e_mean[6];        /* imposed mean strain (Voigt, tensor shear)          */
+} mm_params;
+
+/*
+ * Per-step diagnostics: the fort.98 row (Multi_MEUMAPPS.f:905-906) and the
+ * fort.99 row (:1064) of one step.  glob_min, glob_max, phi_max and
+ * solid_tot lag by one step exactly as in the reference, because the
+ * reference writes fort.98 before its check loop (:1008-1065).
+ */
+typedef struct mm_diag {
+  int32_t step;        /* value of `step` when the row was written          */
+  int32_t n_elastic;   /* elastic fixed-point iterations run in the step    */
+  double fint, fw, fch, elen, etot;
+  double glob_min, glob_max, phi_max, solid_tot;
+  double caviter[MM_MAX_EL];
+  double cfac;         /* final relaxation factor of the elastic iteration  */
+} mm_diag;
+
+typedef struct mm_ctx mm_ctx;
+typedef struct mm_fft mm_fft;
+
+/* ---- whole-step interface: replaces the loop body Multi_MEUMAPPS.f:548-1106 ---- */
+
+/* Allocate device state for one grid.  precision = MM_FP64 or MM_MIXED. */
+int mm_create(const mm_params *p, int precision, mm_ctx **out);
+void mm_destroy(mm_ctx *ctx);
+
+/* Upload the restart record set of data_save.NNNN (Multi_MEUMAPPS.f:473-480):
+ *   phi(V,Nx,Ny,Nz), cons(E,Nx,Ny,Nz), el_en(Nx,Ny,Nz) (may be NULL),
+ *   u = u1|u2|u3 (3 fields), gradu = grad_1_u1, grad_1_u2, grad_1_u3,
+ *   grad_2_u1, ..., grad_3_u3 (9 fields, record order of :478-480).  */
+int mm_upload(mm_ctx *ctx, const double *phi, const double *cons, const double *el_en,
+              const double *u, const double *gradu);
+
+/* Download the same record set (the dump block Multi_MEUMAPPS.f:1087-1098).
+ * Any pointer may be NULL to skip that record. */
+int mm_download(mm_ctx *ctx, double *phi, double *cons, double *el_en, double *u,
+                double *gradu);
+
+/* Advance nsteps steps.  diag (may be NULL) receives nsteps rows.  The
+ * step counter in the context advances like the reference's `step`. */
+int mm_step(mm_ctx *ctx, int32_t nsteps, mm_diag *diag);
+
+int mm_get_step(const mm_ctx *ctx);
+int mm_set_step(mm_ctx *ctx, int32_t step);
+
+/* Device-resident field access for in-process callers (torch, tests):
+ * returns the device address of one SoA field and its element size (8 or 4).
+ * which: 0 phi, 1 cons, 2 el_en, 3 f_diff (last step), 4 strain (6 fields:
+ * e11, e22, e33, 2e23, 2e13, 2e12 incl. e_mean), 5 grad_phi (3V fields),
+ * 6 k_del_sq (V fields), 7 el_grad (V fields). */
+int mm_field(mm_ctx *ctx, int32_t which, void **dev_ptr, int32_t *elem_size,
+             int32_t *nfields);
+
+/* Copy one SoA field group to the host as binary64 (converts in MIXED). */
+int mm_field_to_host(mm_ctx *ctx, int32_t which, double *host);
+
+/* ---- FFT interface: replaces f_trans / inv_trans over P3DFFT (ModFunc.f:15-40) ----
+ * Spectrum layout (device): element (kx, ky, kz), kx in [0, Nx/2],
+ * at index ((kz * (Nx/2+1)) + kx) * Ny + ky, interleaved complex.           */
+int mm_fft_create(int32_t nx, int32_t ny, int32_t nz, int32_t precision, mm_fft **out);
+void mm_fft_destroy(mm_fft *f);
+/* f_trans: X = (1/(Nx Ny Nz)) * sum x exp(-2 pi i k.n / N)  (ModFunc.f:22-23) */
+int mm_fft_forward(mm_fft *f, const void *real_dev, void *spec_dev, void *hip_stream);
+/* inv_trans: unnormalised c2r; the imaginary parts of the kx = 0 and
+ * kx = Nx/2 entries are ignored after the y and z passes (FFTW c2r
+ * semantics, ModFunc.f:37).  The spectrum is overwritten (used as scratch). */
+int mm_fft_inverse(mm_fft *f, void *spec_dev, void *real_dev, void *hip_stream);
+size_t mm_fft_spectrum_elems(const mm_fft *f);
+
+/* ---- host helpers (host-side set-up; no device work) ---- */
+/* Numerical-Recipes ran2 as in ran_2.f:1-42, including the float32-folded
+ * constants; state is kept in *state (35 int32) instead of SAVE storage. */
+double mm_ran2(int32_t *idum, int32_t *state);
+/* Uniform initial fields c = cav + a*(2r-1), phi_v = a*(2r-1) drawn from one
+ * ran2 stream in Fortran order (x fastest; per cell: c_1..c_E then phi_1..V). */
+int mm_ic_uniform(int32_t nx, int32_t ny, int32_t nz, int32_t nvar, int32_t nel,
+                  int32_t iseed, double amp, const double *cav, double *phi,
+                  double *cons);
+
+const char *mm_errstr(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,76 @@
+"""ctypes binding of the C ABI in include/meumapps_ss.h.
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when ``libmeumapps_ss.so`` has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int32, c_size_t, c_void_p
+
+from .params import MMDiag, MMParams
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmeumapps_ss.so")
+
+_lib = None
+
+
+class MMError(RuntimeError):
+    """A nonzero return code of the C ABI."""
+
+    def __init__(self, code: int, where: str):
+        msg = load().mm_errstr(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{where}: error {code}: {msg}")
+        self.code = code
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    D = POINTER(c_double)
+    sigs = {
+        "mm_create": (c_int32, [POINTER(MMParams), c_int32, POINTER(c_void_p)]),
+        "mm_destroy": (None, [c_void_p]),
+        "mm_upload": (c_int32, [c_void_p, D, D, D, D, D]),
+        "mm_download": (c_int32, [c_void_p, D, D, D, D, D]),
+        "mm_step": (c_int32, [c_void_p, c_int32, POINTER(MMDiag)]),
+        "mm_get_step": (c_int32, [c_void_p]),
+        "mm_set_step": (c_int32, [c_void_p, c_int32]),
+        "mm_field": (c_int32, [c_void_p, c_int32, POINTER(c_void_p), POINTER(c_int32), POINTER(c_int32)]),
+        "mm_field_to_host": (c_int32, [c_void_p, c_int32, D]),
+        "mm_fft_create": (c_int32, [c_int32, c_int32, c_int32, c_int32, POINTER(c_void_p)]),
+        "mm_fft_destroy": (None, [c_void_p]),
+        "mm_fft_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+        "mm_fft_inverse": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+        "mm_fft_spectrum_elems": (c_size_t, [c_void_p]),
+        "mm_ran2": (c_double, [POINTER(c_int32), POINTER(c_int32)]),
+        "mm_ic_uniform": (c_int32, [c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, c_double,
+                                    D, D, D]),
+        "mm_errstr": (ctypes.c_char_p, [c_int32]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, where: str) -> None:
+    if rc != 0:
+        raise MMError(rc, where)
+
+
+EXPORTED = [
+    "mm_create", "mm_destroy", "mm_upload", "mm_download", "mm_step", "mm_get_step",
+    "mm_set_step", "mm_field", "mm_field_to_host", "mm_fft_create", "mm_fft_destroy",
+    "mm_fft_forward", "mm_fft_inverse", "mm_fft_spectrum_elems", "mm_ran2", "mm_ic_uniform",
+    "mm_errstr",
+]

@@ -1,0 +1,282 @@
+// fft3d.hip -- 3D real FFT plan, kernel dispatch by line length, and the
+// mm_fft_* entry points that stand in for f_trans / inv_trans (ModFunc.f:15-40).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fft3d.h"
+#include "fft_pass.h"
+#include "mm_common.h"
+
+namespace mm {
+
+bool fft_length_ok(int n) {
+  switch (n) {
+    case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96:
+    case 128: case 192: case 256: case 384: case 512:
+      return true;
+  }
+  return false;
+}
+
+static bool xlen_ok(int l) {
+  switch (l) {
+    case 4: case 6: case 8: case 12: case 16: case 24: case 32: case 48: case 64:
+    case 96: case 128: case 192: case 256:
+      return true;
+  }
+  return false;
+}
+
+void make_twiddles(int n, double* out) {
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int e = 0; e <= n / 2; ++e) {
+    double c, s;
+    long q4 = 4L * e;
+    if (q4 % n == 0) {
+      static const double cc[4] = {1.0, 0.0, -1.0, 0.0}, ss[4] = {0.0, 1.0, 0.0, -1.0};
+      int q = (int)(q4 / n);
+      c = cc[q];
+      s = ss[q];
+    } else {
+      long double a = twopi * (long double)e / (long double)n;
+      c = (double)cosl(a);
+      s = (double)sinl(a);
+    }
+    out[2 * e] = c;
+    out[2 * e + 1] = -s;
+  }
+  for (int e = n / 2 + 1; e < n; ++e) {
+    out[2 * e] = out[2 * (n - e)];
+    out[2 * e + 1] = -out[2 * (n - e) + 1];
+  }
+}
+
+static int upload_tw(int n, int prec, void** dst) {
+  std::vector<double> d(2 * (size_t)n);
+  make_twiddles(n, d.data());
+  size_t bytes;
+  std::vector<float> f;
+  const void* src;
+  if (prec == MM_FP64) {
+    bytes = d.size() * sizeof(double);
+    src = d.data();
+  } else {
+    f.resize(d.size());
+    for (size_t i = 0; i < d.size(); ++i) f[i] = (float)d[i];
+    bytes = f.size() * sizeof(float);
+    src = f.data();
+  }
+  MM_HIP(hipMalloc(dst, bytes));
+  MM_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int Fft3::init(int nx_, int ny_, int nz_, int prec_) {
+  if (!fft_length_ok(nx_) || !fft_length_ok(ny_) || !fft_length_ok(nz_)) return MM_ERR_ARG;
+  if (!xlen_ok(nx_ / 2)) return MM_ERR_ARG;
+  if (prec_ != MM_FP64 && prec_ != MM_MIXED) return MM_ERR_ARG;
+  nx = nx_; ny = ny_; nz = nz_; H = nx / 2 + 1; prec = prec_;
+  MM_TRY(upload_tw(nx / 2, prec, &twLx));
+  MM_TRY(upload_tw(nx, prec, &twNx));
+  MM_TRY(upload_tw(ny, prec, &twNy));
+  MM_TRY(upload_tw(nz, prec, &twNz));
+  return 0;
+}
+
+void Fft3::destroy() {
+  void* p[4] = {twLx, twNx, twNy, twNz};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  twLx = twNx = twNy = twNz = nullptr;
+}
+
+// ------------------------------------------------------------- dispatch
+#define MM_XLENS(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+#define MM_NLENS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+template <typename T>
+static Batch<T> make_batch(const void* const* in, void* const* out, int nf) {
+  Batch<T> b;
+  for (int i = 0; i < kMaxBatch; ++i) { b.in[i] = nullptr; b.out[i] = nullptr; }
+  for (int i = 0; i < nf; ++i) { b.in[i] = in[i]; b.out[i] = out[i]; }
+  return b;
+}
+
+template <typename T, int L>
+static void go_x_r2c(const Batch<T>& b, int nf, const void* twL, const void* twN, int ny, int nz,
+                     T scale, hipStream_t s) {
+  using TL = TileLM<T, L>;
+  dim3 grid((ny + TL::B - 1) / TL::B, nz, nf);
+  hipLaunchKernelGGL((k_x_r2c<T, L>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)twL,
+                     (const C2<T>*)twN, ny, scale);
+}
+template <typename T, int L>
+static void go_x_c2r(const Batch<T>& b, int nf, const void* twL, const void* twN, int ny, int nz,
+                     hipStream_t s) {
+  using TL = TileLM<T, L>;
+  dim3 grid((ny + TL::B - 1) / TL::B, nz, nf);
+  hipLaunchKernelGGL((k_x_c2r<T, L>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)twL,
+                     (const C2<T>*)twN, ny);
+}
+template <typename T, int L>
+static void go_y(const Batch<T>& b, int nf, const void* tw, int nlines, int dir, hipStream_t s) {
+  using TL = TileLM<T, L>;
+  dim3 grid((nlines + TL::B - 1) / TL::B, 1, nf);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_y_pass<T, L, -1>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)tw, nlines);
+  else
+    hipLaunchKernelGGL((k_y_pass<T, L, 1>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)tw, nlines);
+}
+template <typename T, int L>
+static void go_z(const Batch<T>& b, int nf, const void* tw, int ncols, int dir, hipStream_t s) {
+  using TE = TileEM<T, L>;
+  dim3 grid((ncols + TE::B - 1) / TE::B, 1, nf);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_z_pass<T, L, -1>), grid, dim3(TE::THREADS), 0, s, b, (const C2<T>*)tw, ncols);
+  else
+    hipLaunchKernelGGL((k_z_pass<T, L, 1>), grid, dim3(TE::THREADS), 0, s, b, (const C2<T>*)tw, ncols);
+}
+
+template <typename T>
+int Fft3::x_r2c(const T* const* in, void* const* out, int nf, hipStream_t s, T scale) const {
+  if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
+  Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
+  switch (nx / 2) {
+#define X(L_) case L_: go_x_r2c<T, L_>(b, nf, twLx, twNx, ny, nz, scale, s); break;
+    MM_XLENS(X)
+#undef X
+    default: return MM_ERR_ARG;
+  }
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int Fft3::x_c2r(void* const* in, T* const* out, int nf, hipStream_t s) const {
+  if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
+  Batch<T> b = make_batch<T>((const void* const*)in, (void* const*)out, nf);
+  switch (nx / 2) {
+#define X(L_) case L_: go_x_c2r<T, L_>(b, nf, twLx, twNx, ny, nz, s); break;
+    MM_XLENS(X)
+#undef X
+    default: return MM_ERR_ARG;
+  }
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int Fft3::y_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t s) const {
+  if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
+  Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
+  int nlines = nz * H;
+  switch (ny) {
+#define X(L_) case L_: go_y<T, L_>(b, nf, twNy, nlines, dir, s); break;
+    MM_NLENS(X)
+#undef X
+    default: return MM_ERR_ARG;
+  }
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int Fft3::z_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t s) const {
+  if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
+  Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
+  int ncols = H * ny;
+  switch (nz) {
+#define X(L_) case L_: go_z<T, L_>(b, nf, twNz, ncols, dir, s); break;
+    MM_NLENS(X)
+#undef X
+    default: return MM_ERR_ARG;
+  }
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int Fft3::forward(const T* const* in, void* const* out, int nf, hipStream_t s, T scale) const {
+  MM_TRY(x_r2c<T>(in, out, nf, s, scale));
+  MM_TRY(y_pass<T>(out, out, nf, -1, s));
+  MM_TRY(z_pass<T>(out, out, nf, -1, s));
+  return 0;
+}
+
+template <typename T>
+int Fft3::inverse(void* const* spec, T* const* out, int nf, hipStream_t s) const {
+  MM_TRY(z_pass<T>(spec, spec, nf, +1, s));
+  MM_TRY(y_pass<T>(spec, spec, nf, +1, s));
+  MM_TRY(x_c2r<T>(spec, out, nf, s));
+  return 0;
+}
+
+template int Fft3::forward<double>(const double* const*, void* const*, int, hipStream_t, double) const;
+template int Fft3::forward<float>(const float* const*, void* const*, int, hipStream_t, float) const;
+template int Fft3::inverse<double>(void* const*, double* const*, int, hipStream_t) const;
+template int Fft3::inverse<float>(void* const*, float* const*, int, hipStream_t) const;
+template int Fft3::x_r2c<double>(const double* const*, void* const*, int, hipStream_t, double) const;
+template int Fft3::x_r2c<float>(const float* const*, void* const*, int, hipStream_t, float) const;
+template int Fft3::x_c2r<double>(void* const*, double* const*, int, hipStream_t) const;
+template int Fft3::x_c2r<float>(void* const*, float* const*, int, hipStream_t) const;
+template int Fft3::y_pass<double>(void* const*, void* const*, int, int, hipStream_t) const;
+template int Fft3::y_pass<float>(void* const*, void* const*, int, int, hipStream_t) const;
+template int Fft3::z_pass<double>(void* const*, void* const*, int, int, hipStream_t) const;
+template int Fft3::z_pass<float>(void* const*, void* const*, int, int, hipStream_t) const;
+
+}  // namespace mm
+
+// ---------------------------------------------------------------- C ABI
+struct mm_fft {
+  mm::Fft3 f;
+};
+
+extern "C" int mm_fft_create(int32_t nx, int32_t ny, int32_t nz, int32_t precision, mm_fft** out) {
+  if (!out) return MM_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MM_ERR_NODEV;
+  mm_fft* h = new mm_fft();
+  int rc = h->f.init(nx, ny, nz, precision);
+  if (rc) {
+    h->f.destroy();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+extern "C" void mm_fft_destroy(mm_fft* h) {
+  if (!h) return;
+  h->f.destroy();
+  delete h;
+}
+
+extern "C" size_t mm_fft_spectrum_elems(const mm_fft* h) { return h ? h->f.ncplx() : 0; }
+
+extern "C" int mm_fft_forward(mm_fft* h, const void* real_dev, void* spec_dev, void* stream) {
+  if (!h || !real_dev || !spec_dev) return MM_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const double n3 = (double)(float)((long)h->f.nx * h->f.ny * h->f.nz);
+  void* out[1] = {spec_dev};
+  if (h->f.prec == MM_FP64) {
+    const double* in[1] = {(const double*)real_dev};
+    return h->f.forward<double>(in, out, 1, s, 1.0 / n3);
+  }
+  const float* in[1] = {(const float*)real_dev};
+  return h->f.forward<float>(in, out, 1, s, (float)(1.0 / n3));
+}
+
+extern "C" int mm_fft_inverse(mm_fft* h, void* spec_dev, void* real_dev, void* stream) {
+  if (!h || !real_dev || !spec_dev) return MM_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  void* sp[1] = {spec_dev};
+  if (h->f.prec == MM_FP64) {
+    double* out[1] = {(double*)real_dev};
+    return h->f.inverse<double>(sp, out, 1, s);
+  }
+  float* out[1] = {(float*)real_dev};
+  return h->f.inverse<float>(sp, out, 1, s);
+}

@@ -1,0 +1,202 @@
+// fft_pass.h -- the three axis passes of the 3D real FFT.
+//
+// Real fields are (Nx, Ny, Nz) with x fastest.  Spectra use the layout
+//   S[kz][kx][ky]   (ky fastest, kx in [0, Nx/2])
+// so that the y pass works on contiguous lines and the z pass on columns
+// that are contiguous across (kx, ky).  The x pass does the transpose
+// between the two layouts on chip: it reads B whole real rows and writes, for
+// every kx, the B consecutive y positions of the transposed plane.
+//
+//   forward:  x_r2c (real rows -> S, scaled) -> y (c2c) -> z (c2c)
+//   inverse:  z (c2c) -> y (c2c) -> x_c2r (S -> real rows)
+//
+// x_r2c packs a real row of Nx = 2L points as L complex points
+// z_n = x_{2n} + i x_{2n+1}, runs an L-point FFT and untangles
+//   X_k = (Z_k + conj Z_{L-k})/2 + w_N^k (Z_k - conj Z_{L-k})/(2i),  k = 0..L.
+// x_c2r inverts that with the imaginary parts of X_0 and X_L ignored (the
+// c2r semantics of FFTW that P3DFFT's btran_c2r inherits).
+#pragma once
+#include "fft_core.h"
+
+namespace mm {
+
+constexpr int kMaxBatch = 16;
+
+template <typename T> struct Batch {
+  const void* in[kMaxBatch];
+  void* out[kMaxBatch];
+};
+
+__host__ __device__ constexpr int pow2floor(int x) {
+  int p = 1;
+  while (p * 2 <= x) p *= 2;
+  return p;
+}
+__host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+constexpr int kLdsBudget = 65536;
+constexpr int kMaxThreads = 256;
+
+// lines per workgroup, line-major LDS (x and y passes)
+template <typename T, int L> struct TileLM {
+  static constexpr int TPL = Plan<L>::TPL;
+  static constexpr int B = pow2floor(cmin(kMaxThreads / TPL, kLdsBudget / (line_pitch<L>() * (int)sizeof(C2<T>))));
+  static constexpr int THREADS = B * TPL;
+  static constexpr int LDS = B * line_pitch<L>();
+};
+// columns per workgroup, element-major LDS (z pass)
+template <typename T, int L> struct TileEM {
+  static constexpr int TPL = Plan<L>::TPL;
+  static constexpr int B = pow2floor(cmin(kMaxThreads / TPL, kLdsBudget / (L * (int)sizeof(C2<T>))));
+  static constexpr int THREADS = B * TPL;
+  static constexpr int LDS = B * L;
+};
+
+// ------------------------------------------------------------------ x r2c
+// grid (ceil(ny/B), nz, nfields); L = Nx/2
+template <typename T, int L>
+__global__ void __launch_bounds__((TileLM<T, L>::THREADS))
+k_x_r2c(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T scale) {
+  using P = Plan<L>;
+  constexpr int TPL = P::TPL, E = P::E, B = TileLM<T, L>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, L>::LDS];
+  const int f = blockIdx.z, z = blockIdx.y, y0 = blockIdx.x * B, t = threadIdx.x;
+  const T* __restrict__ in = (const T*)bt.in[f];
+  C2<T>* __restrict__ out = (C2<T>*)bt.out[f];
+  {
+    const int b = t / TPL, tl = t % TPL, y = y0 + b;
+    C2<T> v[E];
+    if (y < ny) {
+      const C2<T>* row = (const C2<T>*)(in + ((size_t)z * ny + y) * (size_t)(2 * L));
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = row[tl + TPL * q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = mk<T>(0, 0);
+    }
+    IdxLineMajor idx{b * LP};
+    line_fft<T, -1, P>(v, tl, lds, idx, twL);
+    if constexpr (P::NS > 1) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) lds[idx(tl + TPL * q)] = v[q];
+  }
+  __syncthreads();
+  {
+    const int b = t % B, r = t / B, y = y0 + b;
+    if (y < ny) {
+      const T hs = (T)0.5 * scale;
+      for (int kx = r; kx <= L; kx += TPL) {
+        const int k1 = kx == L ? 0 : kx, k2 = kx == 0 ? 0 : L - kx;
+        C2<T> zk = lds[b * LP + k1 + (k1 >> 4)];
+        C2<T> zc = lds[b * LP + k2 + (k2 >> 4)];
+        // E = (zk + conj zc)/2 ; O = (zk - conj zc)/(2i)
+        C2<T> ev = mk<T>(zk.x + zc.x, zk.y - zc.y);
+        C2<T> od = mk<T>(zk.y + zc.y, -(zk.x - zc.x));
+        C2<T> w = twN[kx];
+        C2<T> x = cadd<T>(ev, cmul<T>(w, od));
+        out[((size_t)z * (L + 1) + kx) * ny + y] = cscale<T>(x, hs);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ x c2r
+template <typename T, int L>
+__global__ void __launch_bounds__((TileLM<T, L>::THREADS))
+k_x_c2r(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny) {
+  using P = Plan<L>;
+  constexpr int TPL = P::TPL, E = P::E, B = TileLM<T, L>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, L>::LDS];
+  const int f = blockIdx.z, z = blockIdx.y, y0 = blockIdx.x * B, t = threadIdx.x;
+  const C2<T>* __restrict__ in = (const C2<T>*)bt.in[f];
+  T* __restrict__ out = (T*)bt.out[f];
+  {
+    const int b = t % B, r = t / B, y = y0 + b;
+    for (int kx = r; kx <= L; kx += TPL) {
+      C2<T> x = (y < ny) ? in[((size_t)z * (L + 1) + kx) * ny + y] : mk<T>(0, 0);
+      lds[b * LP + kx + (kx >> 4)] = x;
+    }
+  }
+  __syncthreads();
+  const int b = t / TPL, tl = t % TPL, y = y0 + b;
+  IdxLineMajor idx{b * LP};
+  C2<T> v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int k = tl + TPL * q;
+    C2<T> xk = lds[idx(k)], xc = lds[idx(L - k)];
+    if (k == 0) { xk.y = 0; xc.y = 0; }
+    C2<T> ev = mk<T>(xk.x + xc.x, xk.y - xc.y);  // xk + conj xc
+    C2<T> dd = mk<T>(xk.x - xc.x, xk.y + xc.y);  // xk - conj xc
+    C2<T> w = twN[k];
+    w.y = -w.y;
+    C2<T> od = cmul<T>(dd, w);
+    v[q] = mk<T>(ev.x - od.y, ev.y + od.x);  // ev + i*od
+  }
+  __syncthreads();
+  line_fft<T, +1, P>(v, tl, lds, idx, twL);
+  if (y < ny) {
+    C2<T>* row = (C2<T>*)(out + ((size_t)z * ny + y) * (size_t)(2 * L));
+#pragma unroll
+    for (int q = 0; q < E; ++q) row[tl + TPL * q] = v[q];
+  }
+}
+
+// -------------------------------------------------- y pass (contiguous lines)
+// grid (ceil(nlines/B), 1, nfields); line l = kz*(Nx/2+1) + kx
+template <typename T, int L, int DIR>
+__global__ void __launch_bounds__((TileLM<T, L>::THREADS))
+k_y_pass(Batch<T> bt, const C2<T>* __restrict__ twL, int nlines) {
+  using P = Plan<L>;
+  constexpr int TPL = P::TPL, E = P::E, B = TileLM<T, L>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, L>::LDS];
+  const int f = blockIdx.z, t = threadIdx.x;
+  const int b = t / TPL, tl = t % TPL, line = blockIdx.x * B + b;
+  // in and out may be the same buffer (in-place pass): no __restrict__
+  const C2<T>* in = (const C2<T>*)bt.in[f] + (size_t)line * L;
+  C2<T>* out = (C2<T>*)bt.out[f] + (size_t)line * L;
+  C2<T> v[E];
+  if (line < nlines) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = in[tl + TPL * q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = mk<T>(0, 0);
+  }
+  IdxLineMajor idx{b * LP};
+  line_fft<T, DIR, P>(v, tl, lds, idx, twL);
+  if (line < nlines) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) out[tl + TPL * q] = v[q];
+  }
+}
+
+// ----------------------------------------------------- z pass (strided lines)
+// grid (ceil(ncols/B), 1, nfields); column c = kx*Ny + ky, stride ncols
+template <typename T, int L, int DIR>
+__global__ void __launch_bounds__((TileEM<T, L>::THREADS))
+k_z_pass(Batch<T> bt, const C2<T>* __restrict__ twL, int ncols) {
+  using P = Plan<L>;
+  constexpr int TPL = P::TPL, E = P::E, B = TileEM<T, L>::B;
+  __shared__ C2<T> lds[TileEM<T, L>::LDS];
+  const int f = blockIdx.z, t = threadIdx.x;
+  const int b = t % B, tl = t / B, col = blockIdx.x * B + b;
+  const C2<T>* in = (const C2<T>*)bt.in[f] + col;
+  C2<T>* out = (C2<T>*)bt.out[f] + col;
+  C2<T> v[E];
+  if (col < ncols) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = in[(size_t)(tl + TPL * q) * ncols];
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = mk<T>(0, 0);
+  }
+  IdxElemMajor idx{b, B};
+  line_fft<T, DIR, P>(v, tl, lds, idx, twL);
+  if (col < ncols) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) out[(size_t)(tl + TPL * q) * ncols] = v[q];
+  }
+}
+
+}  // namespace mm

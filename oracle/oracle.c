@@ -1,0 +1,1011 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * CPU binary64 restatement of one MEUMAPPS-SS semi-implicit spectral step.
+ * The reference (Fortran + P3DFFT + FFTW + LINPACK + MPI) cannot be built in
+ * this image: P3DFFT, FFTW3 and LINPACK are absent, so this file restates the
+ * algorithm from the Fortran source, single process, following the
+ * reference's operation order wherever it affects rounding:
+ *
+ *   f_trans / inv_trans ............ ModFunc.f:15-40 (P3DFFT r2c 'fft' + 1/N^3,
+ *                                    c2r 'tff', FFTW c2r semantics)
+ *   invert .......................... ModFunc.f:44-72 (LU with partial pivoting
+ *                                    + inverse, as LINPACK dgeco/dgedi job 11)
+ *   k_space ......................... ModInitials.f:348-409
+ *   gradient block .................. Multi_MEUMAPPS.f:575-615
+ *   KKS / Landau chemistry .......... Multi_MEUMAPPS.f:634-845
+ *   eldis / elener .................. ModElastic.f:11-691
+ *   reductions ...................... Multi_MEUMAPPS.f:885-907
+ *   Cahn-Hilliard update ............ Multi_MEUMAPPS.f:910-949
+ *   Allen-Cahn update ............... Multi_MEUMAPPS.f:955-1001
+ *   clamp + check loop .............. Multi_MEUMAPPS.f:1008-1065
+ *
+ * Deviations that only move bits in the last place: the FFT is this file's
+ * own mixed-radix Cooley-Tukey (not FFTW); the LINPACK rcond estimate is
+ * replaced by the exact 1-norm condition number of the computed inverse;
+ * global sums are taken per z-slab and then over slabs in z order, which is
+ * deterministic for any thread count.  Langevin noise (step <= noise_step)
+ * is not restated (MM_ERR_ARG).
+ */
+#include "oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------ FFT -- */
+typedef struct {
+  int n, nf, f[48];
+  cplx *w; /* w[e] = exp(-2 pi i e / n), e in [0, n) */
+} plan1d;
+
+static int plan_init(plan1d *p, int n) {
+  p->n = n;
+  p->nf = 0;
+  int m = n;
+  while (m % 4 == 0) { p->f[p->nf++] = 4; m /= 4; }
+  while (m % 2 == 0) { p->f[p->nf++] = 2; m /= 2; }
+  while (m % 3 == 0) { p->f[p->nf++] = 3; m /= 3; }
+  while (m % 5 == 0) { p->f[p->nf++] = 5; m /= 5; }
+  if (m != 1) p->f[p->nf++] = m; /* remaining prime: direct DFT */
+  p->w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  if (!p->w) return -1;
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int e = 0; e < n; ++e) {
+    long double a = twopi * (long double)e / (long double)n;
+    p->w[e] = (double)cosl(a) - I * (double)sinl(a);
+  }
+  return 0;
+}
+
+static void plan_free(plan1d *p) { free(p->w); p->w = NULL; }
+
+static inline cplx tw(const plan1d *p, long e, int sign) {
+  cplx w = p->w[e % p->n];
+  return sign < 0 ? w : conj(w);
+}
+
+/* DFT of in[0], in[is], ..., in[(n-1) is] into out[0..n); sign -1 forward. */
+static void fft_rec(cplx *out, const cplx *in, int n, int is, const plan1d *p, int fi,
+                    int wstride, int sign) {
+  if (n == 1) { out[0] = in[0]; return; }
+  int r = p->f[fi], m = n / r;
+  for (int q = 0; q < r; ++q) fft_rec(out + q * m, in + q * is, m, is * r, p, fi + 1, wstride * r, sign);
+  cplx t[64];
+  for (int k = 0; k < m; ++k) {
+    t[0] = out[k];
+    for (int q = 1; q < r; ++q) t[q] = out[q * m + k] * tw(p, (long)q * k * wstride, sign);
+    if (r == 2) {
+      out[k] = t[0] + t[1];
+      out[k + m] = t[0] - t[1];
+    } else if (r == 4) {
+      cplx a0 = t[0] + t[2], a1 = t[0] - t[2], a2 = t[1] + t[3], a3 = t[1] - t[3];
+      cplx ja3 = (sign < 0) ? -I * a3 : I * a3;
+      out[k] = a0 + a2;
+      out[k + m] = a1 + ja3;
+      out[k + 2 * m] = a0 - a2;
+      out[k + 3 * m] = a1 - ja3;
+    } else {
+      for (int s = 0; s < r; ++s) {
+        cplx acc = t[0];
+        for (int q = 1; q < r; ++q) acc += t[q] * tw(p, (long)((q * s) % r) * m * wstride, sign);
+        out[k + s * m] = acc;
+      }
+    }
+  }
+}
+
+/* transform `count` lines: line l starts at base + l_off(l), element stride es */
+static void fft_lines(cplx *data, const plan1d *p, int count, long (*off)(int, const int *),
+                      const int *geo, long es, int sign, int nthreads) {
+  int n = p->n;
+#pragma omp parallel num_threads(nthreads)
+  {
+    cplx *a = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *b = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+#pragma omp for schedule(static)
+    for (int l = 0; l < count; ++l) {
+      cplx *base = data + off(l, geo);
+      for (int e = 0; e < n; ++e) a[e] = base[(long)e * es];
+      fft_rec(b, a, n, 1, p, 0, 1, sign);
+      for (int e = 0; e < n; ++e) base[(long)e * es] = b[e];
+    }
+    free(a);
+    free(b);
+  }
+}
+
+/* geo = {H, ny, nz} */
+static long off_y(int l, const int *g) { /* l = z*H + kx ; line along y */
+  int H = g[0], ny = g[1];
+  long z = l / H, kx = l % H;
+  return z * (long)ny * H + kx;
+}
+static long off_z(int l, const int *g) { /* l = y*H + kx ; line along z */
+  return (long)l;
+}
+
+/* ----------------------------------------------------------- small LU inverse */
+/* In-place inverse of the column-major n x n matrix a, Gaussian elimination
+ * with partial pivoting (LINPACK dgefa) followed by the inverse from the
+ * factors (dgedi job 1); the rcond test of ModFunc.f:55 is applied to the
+ * exact 1-norm condition number.  Returns 0 or `flag`. */
+int oracle_invert(double *a, int n, int flag) {
+  if (n <= 0 || n > 32) return flag;
+  int ipvt[32];
+  double anorm = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += fabs(a[i + j * n]);
+    if (s > anorm) anorm = s;
+  }
+  /* dgefa */
+  for (int k = 0; k < n - 1; ++k) {
+    int l = k;
+    double amax = fabs(a[k + k * n]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(a[i + k * n]) > amax) { amax = fabs(a[i + k * n]); l = i; }
+    ipvt[k] = l;
+    if (a[l + k * n] == 0.0) return flag;
+    if (l != k) { double t = a[l + k * n]; a[l + k * n] = a[k + k * n]; a[k + k * n] = t; }
+    double t = -1.0 / a[k + k * n];
+    for (int i = k + 1; i < n; ++i) a[i + k * n] *= t;
+    for (int j = k + 1; j < n; ++j) {
+      double tj = a[l + j * n];
+      if (l != k) { a[l + j * n] = a[k + j * n]; a[k + j * n] = tj; }
+      for (int i = k + 1; i < n; ++i) a[i + j * n] += tj * a[i + k * n];
+    }
+  }
+  ipvt[n - 1] = n - 1;
+  if (a[(n - 1) + (n - 1) * n] == 0.0) return flag;
+  /* dgedi: inverse(U) */
+  for (int k = 0; k < n; ++k) {
+    a[k + k * n] = 1.0 / a[k + k * n];
+    double t = -a[k + k * n];
+    for (int i = 0; i < k; ++i) a[i + k * n] *= t;
+    for (int j = k + 1; j < n; ++j) {
+      double tj = a[k + j * n];
+      a[k + j * n] = 0.0;
+      for (int i = 0; i <= k; ++i) a[i + j * n] += tj * a[i + k * n];
+    }
+  }
+  /* inverse(U) * inverse(L) */
+  double work[32];
+  for (int kb = 0; kb < n - 1; ++kb) {
+    int k = n - 2 - kb;
+    for (int i = k + 1; i < n; ++i) { work[i] = a[i + k * n]; a[i + k * n] = 0.0; }
+    for (int j = k + 1; j < n; ++j) {
+      double tj = work[j];
+      for (int i = 0; i < n; ++i) a[i + k * n] += tj * a[i + j * n];
+    }
+    int l = ipvt[k];
+    if (l != k)
+      for (int i = 0; i < n; ++i) { double t = a[i + k * n]; a[i + k * n] = a[i + l * n]; a[i + l * n] = t; }
+  }
+  double inorm = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += fabs(a[i + j * n]);
+    if (s > inorm) inorm = s;
+  }
+  double rcond = (anorm > 0.0 && inorm > 0.0) ? 1.0 / (anorm * inorm) : 0.0;
+  if (!(rcond > 1.0e-16)) return flag;
+  return 0;
+}
+
+/* ---------------------------------------------------------------- k-space */
+/* ModInitials.f:355-401: kf = i * m * fl(2 pi / N), m wrapped to (-N/2, N/2],
+ * zeroed at m = N/2; kf_sq = kf*kf kept at the Nyquist index. */
+void oracle_kaxis(int n, int count, double *kf_im, double *kq) {
+  double pi = atan(1.0) * 4.0;
+  double unit = (2.0 * pi) / (double)(float)n;
+  for (int m = 0; m < count; ++m) {
+    int mw = (m <= n / 2) ? m : m - n;
+    double a = (double)(float)mw * unit;
+    kf_im[m] = (2 * m == n) ? 0.0 : a;
+    kq[m] = -(a * a);
+  }
+}
+
+/* ----------------------------------------------------------------- oracle */
+struct oracle {
+  mm_params p;
+  int nx, ny, nz, H, V, E, NPH, zdim;
+  long n3, nc;
+  int nthreads;
+  plan1d px, py, pz;
+  double *kfx, *kqx, *kfy, *kqy, *kfz, *kqz;
+  double ooN;
+  int phase_of[MM_MAX_VAR]; /* 1-based phase of each variant */
+  /* state */
+  double *phi, *cons, *u, *gu, *el_en;
+  double glob_min, glob_max, phi_max, solid_frac;
+  /* work */
+  double *gradx, *grady, *gradz, *kdel, *fgrad, *fdiff, *elgrad, *Pphi, *Fx, *Fy, *Fz;
+  double *xf, *u_old;
+  cplx *sphi;  /* V spectra of phi */
+  cplx *sw;    /* 12 work spectra */
+  double *qinv; /* 9 per k (column-major 3x3) */
+  int qinv_ready;
+  int step;
+};
+
+#define RIDX(o, x, y, z) ((long)(x) + (long)(o)->nx * ((long)(y) + (long)(o)->ny * (long)(z)))
+
+static void *xcalloc(long n, size_t sz) { return calloc((size_t)(n > 0 ? n : 1), sz); }
+
+oracle_t *oracle_create(const mm_params *p, int nthreads) {
+  if (p->nx < 2 || p->ny < 2 || p->nz < 2 || (p->nx & 1) || (p->ny & 1) || (p->nz & 1)) return NULL;
+  if (p->vartot < 1 || p->vartot > MM_MAX_VAR || p->numel < 1 || p->numel > MM_MAX_EL ||
+      p->numph < 2 || p->numph > MM_MAX_PH)
+    return NULL;
+  int s = 0;
+  for (int q = 0; q < p->numph; ++q) s += p->nvar[q];
+  if (s != p->vartot + 1 || p->nvar[0] != 1) return NULL;
+  oracle_t *o = (oracle_t *)calloc(1, sizeof(oracle_t));
+  o->p = *p;
+  o->nx = p->nx; o->ny = p->ny; o->nz = p->nz; o->H = p->nx / 2 + 1;
+  o->V = p->vartot; o->E = p->numel; o->NPH = p->numph; o->zdim = p->numph * p->numel;
+  o->n3 = (long)o->nx * o->ny * o->nz;
+  o->nc = (long)o->H * o->ny * o->nz;
+  o->nthreads = nthreads > 0 ? nthreads : 1;
+  o->step = p->step;
+  plan_init(&o->px, o->nx); plan_init(&o->py, o->ny); plan_init(&o->pz, o->nz);
+  o->kfx = malloc(sizeof(double) * o->H); o->kqx = malloc(sizeof(double) * o->H);
+  o->kfy = malloc(sizeof(double) * o->ny); o->kqy = malloc(sizeof(double) * o->ny);
+  o->kfz = malloc(sizeof(double) * o->nz); o->kqz = malloc(sizeof(double) * o->nz);
+  oracle_kaxis(o->nx, o->H, o->kfx, o->kqx);
+  oracle_kaxis(o->ny, o->ny, o->kfy, o->kqy);
+  oracle_kaxis(o->nz, o->nz, o->kfz, o->kqz);
+  o->ooN = 1.0 / (double)(float)((long)o->nx * o->ny * o->nz);
+  {
+    int ivn = 0;
+    for (int ph = 2; ph <= o->NPH; ++ph) {
+      ivn += p->nvar[ph - 2];
+      for (int v = ivn; v <= ivn + p->nvar[ph - 1] - 1; ++v) o->phase_of[v - 1] = ph;
+    }
+  }
+  long n3 = o->n3, V = o->V, E = o->E;
+  o->phi = xcalloc(V * n3, 8); o->cons = xcalloc(E * n3, 8);
+  o->u = xcalloc(3 * n3, 8); o->gu = xcalloc(9 * n3, 8); o->el_en = xcalloc(n3, 8);
+  o->gradx = xcalloc(V * n3, 8); o->grady = xcalloc(V * n3, 8); o->gradz = xcalloc(V * n3, 8);
+  o->kdel = xcalloc(V * n3, 8); o->fgrad = xcalloc(V * n3, 8); o->fdiff = xcalloc(V * n3, 8);
+  o->elgrad = xcalloc(V * n3, 8); o->Pphi = xcalloc(V * n3, 8);
+  o->Fx = xcalloc(E * n3, 8); o->Fy = xcalloc(E * n3, 8); o->Fz = xcalloc(E * n3, 8);
+  o->xf = xcalloc(6 * n3, 8); o->u_old = xcalloc(3 * n3, 8);
+  o->sphi = xcalloc(V * o->nc, sizeof(cplx));
+  o->sw = xcalloc(12 * o->nc, sizeof(cplx));
+  o->qinv = xcalloc(9 * o->nc, 8);
+  o->glob_min = 1.0; o->glob_max = 0.0; o->phi_max = 0.0; o->solid_frac = 0.0;
+  if (!o->phi || !o->sw || !o->qinv) { oracle_destroy(o); return NULL; }
+  return o;
+}
+
+void oracle_destroy(oracle_t *o) {
+  if (!o) return;
+  plan_free(&o->px); plan_free(&o->py); plan_free(&o->pz);
+  double *bufs[] = {o->kfx, o->kqx, o->kfy, o->kqy, o->kfz, o->kqz, o->phi, o->cons, o->u,
+                    o->gu, o->el_en, o->gradx, o->grady, o->gradz, o->kdel, o->fgrad,
+                    o->fdiff, o->elgrad, o->Pphi, o->Fx, o->Fy, o->Fz, o->xf, o->u_old, o->qinv};
+  for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) free(bufs[i]);
+  free(o->sphi); free(o->sw);
+  free(o);
+}
+
+int oracle_set_threads(oracle_t *o, int nthreads) {
+  o->nthreads = nthreads > 0 ? nthreads : 1;
+  return 0;
+}
+
+int oracle_get_step(oracle_t *o) { return o->step; }
+void oracle_set_step(oracle_t *o, int step) { o->step = step; }
+
+int oracle_set_state(oracle_t *o, const double *phi, const double *cons, const double *u,
+                     const double *gradu, const double *el_en) {
+  long n3 = o->n3;
+  if (phi) memcpy(o->phi, phi, sizeof(double) * o->V * n3);
+  if (cons) memcpy(o->cons, cons, sizeof(double) * o->E * n3);
+  if (u) memcpy(o->u, u, sizeof(double) * 3 * n3);
+  if (gradu) memcpy(o->gu, gradu, sizeof(double) * 9 * n3);
+  if (el_en) memcpy(o->el_en, el_en, sizeof(double) * n3);
+  return 0;
+}
+
+int oracle_get_state(oracle_t *o, double *phi, double *cons, double *u, double *gradu,
+                     double *el_en) {
+  long n3 = o->n3;
+  if (phi) memcpy(phi, o->phi, sizeof(double) * o->V * n3);
+  if (cons) memcpy(cons, o->cons, sizeof(double) * o->E * n3);
+  if (u) memcpy(u, o->u, sizeof(double) * 3 * n3);
+  if (gradu) memcpy(gradu, o->gu, sizeof(double) * 9 * n3);
+  if (el_en) memcpy(el_en, o->el_en, sizeof(double) * n3);
+  return 0;
+}
+
+int oracle_get_work(oracle_t *o, int which, double *out) {
+  long n3 = o->n3, V = o->V, E = o->E;
+  switch (which) {
+    case 0:
+      memcpy(out, o->gradx, 8 * V * n3);
+      memcpy(out + V * n3, o->grady, 8 * V * n3);
+      memcpy(out + 2 * V * n3, o->gradz, 8 * V * n3);
+      return 0;
+    case 1: memcpy(out, o->kdel, 8 * V * n3); return 0;
+    case 2: memcpy(out, o->fgrad, 8 * V * n3); return 0;
+    case 3: memcpy(out, o->fdiff, 8 * V * n3); return 0;
+    case 4: memcpy(out, o->elgrad, 8 * V * n3); return 0;
+    case 5:
+      memcpy(out, o->Fx, 8 * E * n3);
+      memcpy(out + E * n3, o->Fy, 8 * E * n3);
+      memcpy(out + 2 * E * n3, o->Fz, 8 * E * n3);
+      return 0;
+    case 6: memcpy(out, o->Pphi, 8 * V * n3); return 0;
+  }
+  return MM_ERR_ARG;
+}
+
+/* f_trans (ModFunc.f:15-26): r2c over x, c2c over y and z, then * ooNxyz */
+static void f_trans(oracle_t *o, const double *in, cplx *out) {
+  int nx = o->nx, ny = o->ny, nz = o->nz, H = o->H, nt = o->nthreads;
+  int rows = ny * nz;
+#pragma omp parallel num_threads(nt)
+  {
+    cplx *a = (cplx *)malloc(sizeof(cplx) * nx), *b = (cplx *)malloc(sizeof(cplx) * nx);
+#pragma omp for schedule(static)
+    for (int r = 0; r < rows; ++r) {
+      const double *row = in + (long)r * nx;
+      for (int x = 0; x < nx; ++x) a[x] = row[x];
+      fft_rec(b, a, nx, 1, &o->px, 0, 1, -1);
+      cplx *orow = out + (long)r * H;
+      for (int k = 0; k < H; ++k) orow[k] = b[k];
+    }
+    free(a); free(b);
+  }
+  int geo[3] = {H, ny, nz};
+  /* along y: line (z, kx), elements stride H */
+  fft_lines(out, &o->py, nz * H, off_y, geo, H, -1, nt);
+  /* along z: line (y, kx), elements stride H*ny */
+  fft_lines(out, &o->pz, ny * H, off_z, geo, (long)H * ny, -1, nt);
+  double s = o->ooN;
+  long nc = o->nc;
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (long i = 0; i < nc; ++i) out[i] = out[i] * s;
+}
+
+/* inv_trans (ModFunc.f:30-40): c2c over z and y, then c2r over x ignoring
+ * the imaginary parts of the kx = 0 and kx = Nx/2 entries.  `in` is
+ * overwritten. */
+static void inv_trans(oracle_t *o, cplx *in, double *out) {
+  int nx = o->nx, ny = o->ny, nz = o->nz, H = o->H, nt = o->nthreads;
+  int geo[3] = {H, ny, nz};
+  fft_lines(in, &o->pz, ny * H, off_z, geo, (long)H * ny, +1, nt);
+  fft_lines(in, &o->py, nz * H, off_y, geo, H, +1, nt);
+  int rows = ny * nz;
+#pragma omp parallel num_threads(nt)
+  {
+    cplx *a = (cplx *)malloc(sizeof(cplx) * nx), *b = (cplx *)malloc(sizeof(cplx) * nx);
+#pragma omp for schedule(static)
+    for (int r = 0; r < rows; ++r) {
+      const cplx *irow = in + (long)r * H;
+      a[0] = creal(irow[0]);
+      for (int k = 1; k < H; ++k) a[k] = irow[k];
+      if ((nx & 1) == 0) a[nx / 2] = creal(irow[nx / 2]);
+      for (int k = H; k < nx; ++k) a[k] = conj(a[nx - k]);
+      fft_rec(b, a, nx, 1, &o->px, 0, 1, +1);
+      double *row = out + (long)r * nx;
+      for (int x = 0; x < nx; ++x) row[x] = creal(b[x]);
+    }
+    free(a); free(b);
+  }
+}
+
+int oracle_f_trans(oracle_t *o, const double *in, double *spec) {
+  f_trans(o, in, (cplx *)spec);
+  return 0;
+}
+
+int oracle_inv_trans(oracle_t *o, const double *spec, double *out) {
+  cplx *tmp = (cplx *)malloc(sizeof(cplx) * o->nc);
+  memcpy(tmp, spec, sizeof(cplx) * o->nc);
+  inv_trans(o, tmp, out);
+  free(tmp);
+  return 0;
+}
+
+/* kf and kf_sq at spectrum index (kx, ky, kz) */
+#define KX(o, i) ((o)->kfx[i])
+#define GC(o, v, i, j) ((o)->p.grad_coeff[(v) * 9 + (j) * 3 + (i)]) /* 0-based */
+#define TP(o, m, v) ((o)->p.trans_ppt[(v) * 6 + (m)])
+#define CE(o, i, j, ph) ((o)->p.cela[((ph) - 1) * 36 + (j) * 6 + (i)]) /* ph 1-based */
+#define SE(o, i, j, ph) ((o)->p.sela[((ph) - 1) * 36 + (j) * 6 + (i)])
+#define GP(o, j, ph) ((o)->p.gparams[((ph) - 1) * MM_GP_STRIDE + ((j) - 1)]) /* 1-based */
+
+static inline cplx mul_i(cplx z, double a) { /* z * (i a) */
+  return (-cimag(z) * a) + I * (creal(z) * a);
+}
+
+/* the 2-form  sum G_ij k_i k_j  of Multi_MEUMAPPS.f:596-600 / :979-983 */
+static inline double kgk(const oracle_t *o, int v, int kx, int ky, int kz) {
+  double k1q = o->kqx[kx], k2q = o->kqy[ky], k3q = o->kqz[kz];
+  double a1 = o->kfx[kx], a2 = o->kfy[ky], a3 = o->kfz[kz];
+  double k1k2 = -(a1 * a2), k2k3 = -(a2 * a3), k3k1 = -(a3 * a1);
+  return GC(o, v, 0, 0) * k1q + GC(o, v, 1, 1) * k2q + GC(o, v, 2, 2) * k3q +
+         (GC(o, v, 0, 1) + GC(o, v, 1, 0)) * k1k2 + (GC(o, v, 1, 2) + GC(o, v, 2, 1)) * k2k3 +
+         (GC(o, v, 2, 0) + GC(o, v, 0, 2)) * k3k1;
+}
+
+/* gradient block, Multi_MEUMAPPS.f:575-615 */
+static void gradients(oracle_t *o) {
+  long n3 = o->n3, nc = o->nc;
+  int H = o->H, ny = o->ny, nt = o->nthreads;
+  for (int v = 0; v < o->V; ++v) {
+    cplx *S = o->sphi + v * nc;
+    f_trans(o, o->phi + v * n3, S);
+    cplx *gx = o->sw, *gy = o->sw + nc, *gz = o->sw + 2 * nc, *kd = o->sw + 3 * nc;
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long i = 0; i < nc; ++i) {
+      int kx = (int)(i % H), ky = (int)((i / H) % ny), kz = (int)(i / ((long)H * ny));
+      cplx s = S[i];
+      gx[i] = mul_i(s, o->kfx[kx]);
+      gy[i] = mul_i(s, o->kfy[ky]);
+      gz[i] = mul_i(s, o->kfz[kz]);
+      kd[i] = s * kgk(o, v, kx, ky, kz);
+    }
+    inv_trans(o, gx, o->gradx + v * n3);
+    inv_trans(o, gy, o->grady + v * n3);
+    inv_trans(o, gz, o->gradz + v * n3);
+    inv_trans(o, kd, o->kdel + v * n3);
+  }
+}
+
+/* per-point chemistry, Multi_MEUMAPPS.f:634-845.  Returns 0 or err. */
+static int chem_point(const oracle_t *o, long pt, double *fint, double *fch, double *fw) {
+  const int V = o->V, E = o->E, NPH = o->NPH, zdim = o->zdim;
+  long n3 = o->n3;
+  double phi[MM_MAX_VAR], Ppr[MM_MAX_VAR], Pph[MM_MAX_PH + 1], Gph[MM_MAX_PH + 1];
+  for (int v = 0; v < V; ++v) phi[v] = o->phi[v * n3 + pt];
+  for (int q = 0; q <= NPH; ++q) { Pph[q] = 0.0; Gph[q] = 0.0; }
+  int ivn = 0;
+  for (int ph = 2; ph <= NPH; ++ph) {
+    ivn += o->p.nvar[ph - 2];
+    int fvn = ivn + o->p.nvar[ph - 1] - 1;
+    for (int iv = ivn; iv <= fvn; ++iv) {
+      int v = iv - 1;
+      double fi = phi[v], fi_sq = fi * fi;
+      double pl = fi_sq * fi * (6.0 * fi_sq - 15.0 * fi + 10.0);
+      o->Pphi[v * n3 + pt] = pl;
+      Ppr[v] = 30.0 * fi_sq * (fi_sq - 2.0 * fi + 1.0);
+      Pph[1] += pl;
+      Pph[ph] += pl;
+      Gph[ph] += fi_sq * ((1.0 - fi) * (1.0 - fi));
+      for (int jv = 0; jv < V; ++jv)
+        if (jv != v) Gph[1] += fi_sq * (phi[jv] * phi[jv]);
+    }
+  }
+  Pph[1] = 1.0 - Pph[1];
+  /* Zintf: constant rows (Multi_MEUMAPPS.f:426-448) + mixture rows (:677-688) */
+  double Z[16 * 16], Zr[16];
+  memset(Z, 0, sizeof(Z));
+  for (int e = 1; e <= E; ++e) {
+    int i1 = e + E + 1;
+    for (int ph = 1; ph <= NPH - 1; ++ph) {
+      int i0 = (NPH - 1) * (e - 1) + ph, j0 = NPH * (e - 1) + ph;
+      Z[(i0 - 1) + (j0 - 1) * zdim] = GP(o, e, ph);
+      Z[(i0 - 1) + j0 * zdim] = -GP(o, e, ph + 1);
+      Zr[i0 - 1] = GP(o, e, ph) * GP(o, i1, ph) - GP(o, e, ph + 1) * GP(o, i1, ph + 1);
+    }
+  }
+  for (int e = 1; e <= E; ++e) {
+    int i0 = zdim - E + e;
+    Zr[i0 - 1] = o->cons[(e - 1) * n3 + pt];
+    for (int ph = 1; ph <= NPH; ++ph) {
+      int j0 = NPH * (e - 1) + ph;
+      Z[(i0 - 1) + (j0 - 1) * zdim] = Pph[ph];
+    }
+  }
+  if (oracle_invert(Z, zdim, MM_ERR_ZSOLVE)) return MM_ERR_ZSOLVE;
+  double X[MM_MAX_EL + 1][MM_MAX_PH + 1];
+  for (int ph = 1; ph <= NPH; ++ph)
+    for (int e = 1; e <= E; ++e) {
+      int i0 = NPH * (e - 1) + ph;
+      double acc = 0.0;
+      for (int j0 = 1; j0 <= zdim; ++j0) acc += Z[(i0 - 1) + (j0 - 1) * zdim] * Zr[j0 - 1];
+      X[e][ph] = acc;
+    }
+  double Gf[MM_MAX_PH + 1] = {0}, Mu[MM_MAX_EL + 1] = {0};
+  for (int ph = 1; ph <= NPH; ++ph) {
+    double g = 0.0;
+    for (int e = 1; e <= E; ++e) {
+      double d = X[e][ph] - GP(o, e + E + 1, ph);
+      g += GP(o, e, ph) * (d * d);
+    }
+    Gf[ph] = g + GP(o, E + 1, ph);
+  }
+  for (int e = 1; e <= E; ++e) Mu[e] = 2.0 * GP(o, e, 1) * (X[e][1] - GP(o, e + E + 1, 1));
+  double fchloc = Pph[1] * Gf[1];
+  double fwloc = o->p.wellph[0] * Gph[1];
+  double Fx[MM_MAX_EL + 1] = {0}, Fy[MM_MAX_EL + 1] = {0}, Fz[MM_MAX_EL + 1] = {0};
+  double fg[MM_MAX_VAR];
+  ivn = 0;
+  for (int ph = 2; ph <= NPH; ++ph) {
+    ivn += o->p.nvar[ph - 2];
+    int fvn = ivn + o->p.nvar[ph - 1] - 1;
+    for (int iv = ivn; iv <= fvn; ++iv) {
+      int v = iv - 1;
+      double gx = o->gradx[v * n3 + pt], gy = o->grady[v * n3 + pt], gz = o->gradz[v * n3 + pt];
+      double term = GC(o, v, 0, 0) * (gx * gx) + GC(o, v, 1, 1) * (gy * gy) +
+                    GC(o, v, 2, 2) * (gz * gz) + (GC(o, v, 0, 1) + GC(o, v, 1, 0)) * gx * gy +
+                    (GC(o, v, 0, 2) + GC(o, v, 2, 0)) * gx * gz +
+                    (GC(o, v, 1, 2) + GC(o, v, 2, 1)) * gy * gz;
+      *fint += 0.5 * term;
+      double fi = phi[v];
+      double termch = Gf[ph] - Gf[1];
+      for (int e = 1; e <= E; ++e) {
+        double xd = X[e][1] - X[e][ph];
+        termch = termch + Mu[e] * xd;
+        double tprc = Ppr[v] * xd;
+        Fx[e] += tprc * gx;
+        Fy[e] += tprc * gy;
+        Fz[e] += tprc * gz;
+      }
+      double wg = 2.0 * o->p.wellph[ph - 1] * fi * (1.0 - fi) * (1.0 - 2.0 * fi);
+      for (int jv = 0; jv < V; ++jv)
+        if (jv != v) wg = wg + 4.0 * o->p.wellph[0] * fi * (phi[jv] * phi[jv]);
+      fg[v] = termch * Ppr[v] + wg;
+    }
+    fchloc = fchloc + Pph[ph] * Gf[ph];
+    fwloc = fwloc + o->p.wellph[ph - 1] * Gph[ph];
+  }
+  *fch += fchloc;
+  *fw += fwloc;
+  for (int e = 1; e <= E; ++e) {
+    o->Fx[(e - 1) * n3 + pt] = Fx[e];
+    o->Fy[(e - 1) * n3 + pt] = Fy[e];
+    o->Fz[(e - 1) * n3 + pt] = Fz[e];
+  }
+  /* f_diff correction, :824-841 */
+  for (int jv = 0; jv < V; ++jv) {
+    double t11 = 0.0;
+    int nt = 1;
+    if (phi[jv] > 0.0) {
+      for (int v = 0; v < V; ++v)
+        if (phi[v] > 0.0 && v != jv) {
+          t11 = t11 + fg[v] - o->kdel[v * n3 + pt];
+          nt += 1;
+        }
+    }
+    o->fgrad[jv * n3 + pt] = fg[jv];
+    o->fdiff[jv * n3 + pt] = fg[jv] - t11 / (double)nt;
+  }
+  return 0;
+}
+
+static int chemistry(oracle_t *o, double *fint, double *fch, double *fw) {
+  int nz = o->nz, nt = o->nthreads;
+  long slab = (long)o->nx * o->ny;
+  double *pf = calloc(3 * nz, sizeof(double));
+  int err = 0;
+#pragma omp parallel for num_threads(nt) schedule(static) reduction(| : err)
+  for (int z = 0; z < nz; ++z) {
+    double a = 0, b = 0, c = 0;
+    for (long i = 0; i < slab; ++i) err |= chem_point(o, z * slab + i, &a, &b, &c);
+    pf[3 * z] = a; pf[3 * z + 1] = b; pf[3 * z + 2] = c;
+  }
+  double a = 0, b = 0, c = 0;
+  for (int z = 0; z < nz; ++z) { a += pf[3 * z]; b += pf[3 * z + 1]; c += pf[3 * z + 2]; }
+  free(pf);
+  *fint = a; *fch = b; *fw = c;
+  return err ? MM_ERR_ZSOLVE : 0;
+}
+
+/* Voigt index of the tensor pair (i,j), 0-based in and out: ModElastic.f:124-126 */
+static inline int voigt(int i, int j) { return (i == j) ? i : (9 - (i + 1) - (j + 1)) - 1; }
+
+/* time-invariant acoustic-tensor inverse Qinv(k), ModElastic.f:115-191 */
+static int build_qinv(oracle_t *o) {
+  long nc = o->nc;
+  int H = o->H, ny = o->ny;
+  int err = 0;
+#pragma omp parallel for num_threads(o->nthreads) schedule(static) reduction(| : err)
+  for (long idx = 0; idx < nc; ++idx) {
+    int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+    if (kx == 0 && ky == 0 && kz == 0) continue;
+    double kf[3] = {o->kfx[kx], o->kfy[ky], o->kfz[kz]};
+    double kq[3] = {o->kqx[kx], o->kqy[ky], o->kqz[kz]};
+    double Q[9] = {0};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        int ij = voigt(i, j);
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            int kl = voigt(k, l);
+            double kpr = (k == j) ? kq[k] : -(kf[k] * kf[j]);
+            Q[i + 3 * l] += CE(o, ij, kl, 2) * kpr;
+          }
+      }
+    if (oracle_invert(Q, 3, MM_ERR_QINV)) { err |= 1; continue; }
+    memcpy(o->qinv + 9 * idx, Q, sizeof(Q));
+  }
+  if (err) return MM_ERR_QINV;
+  o->qinv_ready = 1;
+  return 0;
+}
+
+/* elener, ModElastic.f:493-691 */
+static int elener(oracle_t *o) {
+  const int V = o->V, NPH = o->NPH;
+  long n3 = o->n3;
+  double transsum[MM_MAX_VAR];
+  for (int v = 0; v < V; ++v) {
+    double s = 0.0;
+    for (int m = 0; m < 6; ++m) s += TP(o, m, v);
+    transsum[v] = s;
+  }
+  const double thr = (double)1.e-9f;
+  int err = 0;
+#pragma omp parallel for num_threads(o->nthreads) schedule(static) reduction(| : err)
+  for (long pt = 0; pt < n3; ++pt) {
+    double te[6] = {0}, ps = 0.0, Ppr[MM_MAX_VAR];
+    for (int v = 0; v < V; ++v) {
+      double fi = o->phi[v * n3 + pt], fi_sq = fi * fi;
+      double pl = o->Pphi[v * n3 + pt];
+      ps += pl;
+      Ppr[v] = 30.0 * fi_sq * (fi_sq - 2.0 * fi + 1.0);
+      for (int m = 0; m < 6; ++m) te[m] += pl * TP(o, m, v);
+    }
+    const double *g = o->gu; /* g[(j)*3+(i)] = d_j u_i (0-based) */
+#define GU(i, j) g[((j) * 3 + (i)) * n3 + pt]
+    double eps[6];
+    eps[0] = o->p.e_mean[0] + GU(0, 0) - te[0];
+    eps[1] = o->p.e_mean[1] + GU(1, 1) - te[1];
+    eps[2] = o->p.e_mean[2] + GU(2, 2) - te[2];
+    eps[3] = 2.0 * o->p.e_mean[3] + GU(2, 1) + GU(1, 2) - 2.0 * te[3];
+    eps[4] = 2.0 * o->p.e_mean[4] + GU(2, 0) + GU(0, 2) - 2.0 * te[4];
+    eps[5] = 2.0 * o->p.e_mean[5] + GU(0, 1) + GU(1, 0) - 2.0 * te[5];
+#undef GU
+    double C[36] = {0};
+    int ivn = 0;
+    for (int ph = 2; ph <= NPH; ++ph) {
+      ivn += o->p.nvar[ph - 2];
+      for (int iv = ivn; iv <= ivn + o->p.nvar[ph - 1] - 1; ++iv) {
+        double pl = o->Pphi[(iv - 1) * n3 + pt];
+        for (int q = 0; q < 36; ++q) C[q] = C[q] + pl * o->p.sela[(ph - 1) * 36 + q];
+      }
+    }
+    for (int q = 0; q < 36; ++q) C[q] = C[q] + (1.0 - ps) * o->p.sela[q];
+    if (oracle_invert(C, 6, MM_ERR_CINV_ENER)) { err |= 1; continue; }
+    double sig[6];
+    for (int a = 0; a < 6; ++a) {
+      double s = 0.0;
+      for (int m = 0; m < 6; ++m) s = s + C[a + 6 * m] * eps[m];
+      sig[a] = s;
+    }
+    double el_mat = 0.0;
+    for (int a = 0; a < 6; ++a) {
+      double em = 0.0;
+      for (int m = 0; m < 6; ++m) em = em + SE(o, a, m, 1) * sig[m];
+      el_mat = el_mat + sig[a] * em;
+    }
+    el_mat = 0.5 * el_mat;
+    double een = (1.0 - ps) * el_mat;
+    ivn = 0;
+    for (int ph = 2; ph <= NPH; ++ph) {
+      ivn += o->p.nvar[ph - 2];
+      for (int iv = ivn; iv <= ivn + o->p.nvar[ph - 1] - 1; ++iv) {
+        int v = iv - 1;
+        if (transsum[v] > thr) {
+          double el_ppt = 0.0;
+          for (int a = 0; a < 6; ++a) {
+            double ep = 0.0;
+            for (int m = 0; m < 6; ++m) ep = ep + SE(o, a, m, ph) * sig[m];
+            el_ppt = el_ppt + sig[a] * ep;
+          }
+          el_ppt = 0.5 * el_ppt;
+          een = een + el_ppt * o->Pphi[v * n3 + pt];
+          double eg = (sig[0] * TP(o, 0, v) + sig[1] * TP(o, 1, v) + sig[2] * TP(o, 2, v) +
+                       2.0 * (sig[3] * TP(o, 3, v) + sig[4] * TP(o, 4, v) + sig[5] * TP(o, 5, v)));
+          o->elgrad[v * n3 + pt] = Ppr[v] * (el_mat - el_ppt) - eg * Ppr[v];
+        }
+      }
+    }
+    o->el_en[pt] = een;
+  }
+  return err ? MM_ERR_CINV_ENER : 0;
+}
+
+/* slab-ordered global sums of the six convergence terms, ModElastic.f:413-430 */
+static void err_sums(oracle_t *o, double err[6]) {
+  int nz = o->nz;
+  long slab = (long)o->nx * o->ny, n3 = o->n3;
+  double *pf = calloc(6 * nz, sizeof(double));
+#pragma omp parallel for num_threads(o->nthreads) schedule(static)
+  for (int z = 0; z < nz; ++z) {
+    double e[6] = {0};
+    for (long i = z * slab; i < (z + 1) * slab; ++i)
+      for (int c = 0; c < 3; ++c) {
+        double un = o->u[c * n3 + i], uo = o->u_old[c * n3 + i];
+        e[2 * c] += (un - uo) * (un - uo);
+        e[2 * c + 1] += uo * uo;
+      }
+    memcpy(pf + 6 * z, e, sizeof(e));
+  }
+  for (int c = 0; c < 6; ++c) err[c] = 0.0;
+  for (int z = 0; z < nz; ++z)
+    for (int c = 0; c < 6; ++c) err[c] += pf[6 * z + c];
+  free(pf);
+}
+
+/* eldis, ModElastic.f:11-489 (+ elener) */
+int oracle_eldis(oracle_t *o, int *n_iter, double *cfac_out) {
+  const int V = o->V, NPH = o->NPH, H = o->H, ny = o->ny, nt = o->nthreads;
+  long n3 = o->n3, nc = o->nc;
+  int rc;
+  if (!o->qinv_ready && (rc = build_qinv(o))) return rc;
+  cplx *su = o->sw;           /* 3 spectra of u            */
+  cplx *sg = o->sw + 3 * nc;  /* 9 spectra of grad u       */
+  if (o->step == o->p.elastep0) {
+    /* zeroth order: FFT of P_phi, R with the real*8 truncation of sum_0
+     * (ModElastic.f:29,151), u = Qinv R, u(k=0) = 0 (:105-236) */
+    cplx *sp = (cplx *)malloc(sizeof(cplx) * V * nc);
+    for (int v = 0; v < V; ++v) f_trans(o, o->Pphi + v * n3, sp + v * nc);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long idx = 0; idx < nc; ++idx) {
+      int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+      double kf[3] = {o->kfx[kx], o->kfy[ky], o->kfz[kz]};
+      if (kx == 0 && ky == 0 && kz == 0) {
+        for (int c = 0; c < 3; ++c) su[c * nc + idx] = 0.0;
+        for (int c = 0; c < 9; ++c) sg[c * nc + idx] = 0.0;
+        continue;
+      }
+      double Ri[3] = {0, 0, 0}; /* R(i) = i * Ri[i] */
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          int ij = voigt(i, j);
+          double sum0 = 0.0;
+          for (int k = 0; k < 3; ++k)
+            for (int l = 0; l < 3; ++l) {
+              int kl = voigt(k, l);
+              int ivn = 0;
+              for (int ph = 2; ph <= NPH; ++ph) {
+                ivn += o->p.nvar[ph - 2];
+                for (int iv = ivn; iv <= ivn + o->p.nvar[ph - 1] - 1; ++iv)
+                  sum0 = sum0 + creal(sp[(iv - 1) * nc + idx]) * TP(o, kl, iv - 1) * CE(o, ij, kl, ph);
+              }
+            }
+          Ri[i] = Ri[i] + sum0 * kf[j];
+        }
+      const double *Q = o->qinv + 9 * idx;
+      for (int i = 0; i < 3; ++i) {
+        /* Q(i,1)*R(1) + Q(i,2)*R(2) + Q(i,3)*R(3), R purely imaginary */
+        double ui = Q[i] * Ri[0] + Q[i + 3] * Ri[1] + Q[i + 6] * Ri[2];
+        cplx uh = I * ui;
+        su[i * nc + idx] = uh;
+        for (int d = 0; d < 3; ++d) sg[(d * 3 + i) * nc + idx] = mul_i(uh, kf[d]);
+      }
+    }
+    free(sp);
+    for (int c = 0; c < 3; ++c) inv_trans(o, su + c * nc, o->u + c * n3);
+    for (int c = 0; c < 9; ++c) inv_trans(o, sg + c * nc, o->gu + c * n3);
+  }
+  double newfac = 0.5, cfac = 1.0, err[6], err_old[6] = {0};
+  int niter = 0, converged = 0, total = 0;
+  const int vmap[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+  for (;;) {
+    /* label 2 */
+    memcpy(o->u_old, o->u, sizeof(double) * 3 * n3);
+    niter += 1;
+    total += 1;
+    int ierr = 0;
+#pragma omp parallel for num_threads(nt) schedule(static) reduction(| : ierr)
+    for (long pt = 0; pt < n3; ++pt) {
+      double C[36] = {0}, ps = 0.0;
+      int ivn = 0;
+      for (int ph = 2; ph <= NPH; ++ph) {
+        ivn += o->p.nvar[ph - 2];
+        for (int iv = ivn; iv <= ivn + o->p.nvar[ph - 1] - 1; ++iv) {
+          double pl = o->Pphi[(iv - 1) * n3 + pt];
+          ps = ps + pl;
+          for (int q = 0; q < 36; ++q) C[q] = C[q] + pl * o->p.sela[(ph - 1) * 36 + q];
+        }
+      }
+      if (ps < 1.0)
+        for (int q = 0; q < 36; ++q) C[q] = C[q] + (1.0 - ps) * o->p.sela[q];
+      if (oracle_invert(C, 6, MM_ERR_CINV_ITER)) { ierr |= 1; continue; }
+      double Cp[36];
+      for (int q = 0; q < 36; ++q) Cp[q] = o->p.cela[36 + q] - C[q];
+      double gradu[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) gradu[i][j] = o->gu[(j * 3 + i) * n3 + pt];
+      for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j) {
+          int ij = vmap[i][j];
+          double s = 0.0;
+          for (int k = 0; k < 3; ++k)
+            for (int l = 0; l < 3; ++l) {
+              int kl = vmap[k][l];
+              s = s + cfac * Cp[ij + 6 * kl] * (gradu[k][l] + o->p.e_mean[kl]);
+              for (int v = 0; v < V; ++v)
+                s = s + C[ij + 6 * kl] * o->Pphi[v * n3 + pt] * TP(o, kl, v);
+            }
+          o->xf[ij * n3 + pt] = s;
+        }
+    }
+    if (ierr) return MM_ERR_CINV_ITER;
+    cplx *sx = o->sw; /* 6 spectra; then reused for u (3) and grad u (9) */
+    cplx *sxx = (cplx *)malloc(sizeof(cplx) * 6 * nc);
+    for (int c = 0; c < 6; ++c) f_trans(o, o->xf + c * n3, sxx + c * nc);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long idx = 0; idx < nc; ++idx) {
+      int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+      double kf[3] = {o->kfx[kx], o->kfy[ky], o->kfz[kz]};
+      cplx x1 = sxx[idx], x2 = sxx[nc + idx], x3 = sxx[2 * nc + idx], x4 = sxx[3 * nc + idx],
+           x5 = sxx[4 * nc + idx], x6 = sxx[5 * nc + idx];
+      cplx R[3];
+      R[0] = mul_i(x1, kf[0]) + mul_i(x6, kf[1]) + mul_i(x5, kf[2]);
+      R[1] = mul_i(x6, kf[0]) + mul_i(x2, kf[1]) + mul_i(x4, kf[2]);
+      R[2] = mul_i(x5, kf[0]) + mul_i(x4, kf[1]) + mul_i(x3, kf[2]);
+      cplx uh[3];
+      if (kx == 0 && ky == 0 && kz == 0) {
+        uh[0] = uh[1] = uh[2] = 0.0;
+      } else {
+        const double *Q = o->qinv + 9 * idx;
+        for (int i = 0; i < 3; ++i) uh[i] = Q[i] * R[0] + Q[i + 3] * R[1] + Q[i + 6] * R[2];
+      }
+      for (int i = 0; i < 3; ++i) {
+        sx[i * nc + idx] = uh[i];
+        for (int d = 0; d < 3; ++d) sx[(3 + d * 3 + i) * nc + idx] = mul_i(uh[i], kf[d]);
+      }
+    }
+    free(sxx);
+    for (int c = 0; c < 3; ++c) inv_trans(o, sx + c * nc, o->u + c * n3);
+    for (int c = 0; c < 9; ++c) inv_trans(o, sx + (3 + c) * nc, o->gu + c * n3);
+    err_sums(o, err);
+    if (err[0] / err[1] < 1.e-7 && err[2] / err[3] < 1.e-7 && err[4] / err[5] < 1.e-7) converged = 1;
+    if (niter > 1 && !converged) {
+      if (err[0] / err[1] > err_old[0] / err_old[1] || err[2] / err[3] > err_old[2] / err_old[3] ||
+          err[4] / err[5] > err_old[4] / err_old[5]) {
+        cfac = cfac * newfac;
+        newfac = newfac * 0.5;
+        if (newfac < 1.e-7) return MM_ERR_NEWFAC;
+        niter = 0;
+        converged = 0;
+        continue;
+      }
+    }
+    memcpy(err_old, err, sizeof(err));
+    if (niter == 10) converged = 1;
+    if (!converged || niter < 4) continue;
+    if (cfac < 1.0 - 1.e-6) {
+      cfac = cfac + 0.1;
+      if (cfac > 1.0) cfac = 1.0;
+      niter = 0;
+      converged = 0;
+      continue;
+    }
+    break;
+  }
+  if (n_iter) *n_iter = total;
+  if (cfac_out) *cfac_out = cfac;
+  return elener(o);
+}
+
+int oracle_step(oracle_t *o, mm_diag *d) {
+  const int V = o->V, E = o->E, H = o->H, ny = o->ny, nz = o->nz, nt = o->nthreads;
+  long n3 = o->n3, nc = o->nc;
+  int rc;
+  if (o->step <= o->p.noise_step) return MM_ERR_ARG; /* Langevin noise not restated */
+  mm_diag dd;
+  memset(&dd, 0, sizeof(dd));
+  dd.step = o->step;
+  gradients(o);
+  double fint, fch, fw;
+  if ((rc = chemistry(o, &fint, &fch, &fw))) return rc;
+  memset(o->elgrad, 0, sizeof(double) * V * n3);
+  int niter = 0;
+  double cfac = 1.0;
+  if (o->step >= o->p.elastep0) {
+    if ((rc = oracle_eldis(o, &niter, &cfac))) return rc;
+  } else {
+    memset(o->el_en, 0, sizeof(double) * n3);
+  }
+  for (long i = 0; i < V * n3; ++i) o->fdiff[i] = o->fdiff[i] + o->elgrad[i];
+  double el_sum = 0.0;
+  {
+    long slab = (long)o->nx * o->ny;
+    for (int z = 0; z < nz; ++z) {
+      double s = 0.0;
+      for (long i = z * slab; i < (z + 1) * slab; ++i) s += o->el_en[i];
+      el_sum += s;
+    }
+  }
+  dd.fint = fint; dd.fw = fw; dd.fch = fch; dd.elen = el_sum;
+  dd.etot = fint + fch + el_sum + fw;
+  dd.glob_min = o->glob_min; dd.glob_max = o->glob_max; dd.phi_max = o->phi_max;
+  dd.solid_tot = o->solid_frac / (double)(float)n3;
+  dd.n_elastic = niter;
+  dd.cfac = cfac;
+  /* Cahn-Hilliard, :910-949 */
+  cplx *sc = o->sw, *sfx = o->sw + nc, *sfy = o->sw + 2 * nc, *sfz = o->sw + 3 * nc;
+  for (int e = 0; e < E; ++e) {
+    f_trans(o, o->cons + e * n3, sc);
+    f_trans(o, o->Fx + e * n3, sfx);
+    f_trans(o, o->Fy + e * n3, sfy);
+    f_trans(o, o->Fz + e * n3, sfz);
+    double dtm = o->p.t_step * o->p.cmobs[e];
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long idx = 0; idx < nc; ++idx) {
+      int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+      double ksq = o->kqx[kx] + o->kqy[ky] + o->kqz[kz];
+      cplx t1 = mul_i(sfx[idx], o->kfx[kx]) + mul_i(sfy[idx], o->kfy[ky]) + mul_i(sfz[idx], o->kfz[kz]);
+      double t2 = 1.0 - dtm * ksq;
+      cplx num = sc[idx] + dtm * t1;
+      sc[idx] = creal(num) / t2 + I * (cimag(num) / t2);
+    }
+    inv_trans(o, sc, o->cons + e * n3);
+  }
+  /* Allen-Cahn, :955-1001 (noise term is zero: step > noise_step) */
+  cplx *sf = o->sw + nc;
+  for (int v = 0; v < V; ++v) {
+    cplx *sp = o->sphi + v * nc; /* FFT of phi from the gradient block; phi unchanged since */
+    f_trans(o, o->fdiff + v * n3, sf);
+    double dtl = o->p.t_step * o->p.b_mob;
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long idx = 0; idx < nc; ++idx) {
+      int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+      cplx t1 = dtl * sf[idx];
+      double t2 = 1.0 - dtl * kgk(o, v, kx, ky, kz);
+      cplx num = sp[idx] - t1;
+      sp[idx] = creal(num) / t2 + I * (cimag(num) / t2);
+    }
+    inv_trans(o, sp, o->phi + v * n3);
+  }
+  /* check loop, :1008-1065 */
+  {
+    long slab = (long)o->nx * o->ny;
+    double *pf = calloc((size_t)nz * (4 + E), sizeof(double));
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int z = 0; z < nz; ++z) {
+      double cmin = 1.0, cmax = 0.0, pmax = 0.0, sf = 0.0, cl[MM_MAX_EL] = {0};
+      for (long i = z * slab; i < (z + 1) * slab; ++i) {
+        for (int e = 0; e < E; ++e) {
+          double c = o->cons[e * n3 + i];
+          if (c < cmin) cmin = c;
+          if (c > cmax) cmax = c;
+          cl[e] = cl[e] + c;
+        }
+        for (int v = 0; v < V; ++v) {
+          double f = o->phi[v * n3 + i];
+          if (f < 0.0) f = 0.0;
+          if (f > 1.0) f = 1.0;
+          o->phi[v * n3 + i] = f;
+          if (f > pmax) pmax = f;
+          if (f > 0.0) sf = sf + f;
+        }
+      }
+      double *q = pf + (size_t)z * (4 + E);
+      q[0] = cmin; q[1] = cmax; q[2] = pmax; q[3] = sf;
+      for (int e = 0; e < E; ++e) q[4 + e] = cl[e];
+    }
+    double cmin = 1.0, cmax = 0.0, pmax = 0.0, sfr = 0.0, cl[MM_MAX_EL] = {0};
+    for (int z = 0; z < nz; ++z) {
+      double *q = pf + (size_t)z * (4 + E);
+      if (q[0] < cmin) cmin = q[0];
+      if (q[1] > cmax) cmax = q[1];
+      if (q[2] > pmax) pmax = q[2];
+      sfr += q[3];
+      for (int e = 0; e < E; ++e) cl[e] += q[4 + e];
+    }
+    free(pf);
+    for (int e = 0; e < E; ++e) dd.caviter[e] = cl[e] * o->ooN;
+    o->glob_min = cmin; o->glob_max = cmax; o->phi_max = pmax; o->solid_frac = sfr;
+  }
+  o->step += 1;
+  if (d) *d = dd;
+  return 0;
+}
