@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s of the MEUMAPPS-SS semi-implicit spectral step.
+
+Workload: 256^3 periodic grid, one composition c + 3 tetragonal variants
+(eigenstrain diag(0.02,-0.01,-0.01) permuted per variant), cubic
+C11/C12/C44 = 250/150/100, dt = 0.005, initial fields c = 0.4 + U[-0.01,0.01],
+eta_v = U[-0.01,0.01] from ran_2 with seed 7.  One "step" is one full
+semi-implicit step (gradients, KKS chemistry, elastic fixed-point solve,
+elastic energy, Cahn-Hilliard and Allen-Cahn updates, clamp and
+diagnostics) on inputs resident in HBM.  The elastep0 step (zeroth-order
+elastic solve) runs before the timed region.
+
+Reported: FP64 cell-updates/s as `value`, the mixed FP32/FP64 rate and its
+relative L2 error against FP64 on the same steps, the live HIP-event timing of
+the dominant kernel against the HBM roofline, and the CPU oracle timed on this
+host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def step_bytes_per_cell(V, E, n, w):
+    """Algorithmic bytes per cell-update: one read + write per minimal 3D FFT
+    and one traversal per fused pointwise pass (N_FFT = 7V+5E+15n,
+    N_pw = 8V+5E+6+n(V+18))."""
+    nfft = 7 * V + 5 * E + 15 * n
+    npw = 8 * V + 5 * E + 6 + n * (V + 18)
+    return w * (2 * nfft + npw)
+
+
+def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind):
+    import numpy as np
+    import torch
+
+    from meumapps_ss_amd import _lib
+    from meumapps_ss_amd.sim import Simulation
+
+    lib = _lib.load()
+    shape = phi.shape[1:]
+    sim = Simulation(params, precision)
+    sim.upload(phi=phi, cons=cons, u=np.zeros((3, *shape)), gradu=np.zeros((9, *shape)))
+    sim.step(1 + warmup)  # elastep0 step + warm-up
+    _lib.check(lib.mm_probe(sim.ctx, probe_kind), "mm_probe")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows = sim.step(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = ctypes.c_double()
+    nl = ctypes.c_int64()
+    by = ctypes.c_double()
+    _lib.check(lib.mm_probe_read(sim.ctx, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(by)), "probe")
+    state = sim.download()
+    sim.close()
+    return dict(seconds=t1 - t0, rows=rows, probe_ms=ms.value, probe_launches=nl.value,
+                probe_bytes=by.value, state=state)
+
+
+def cpu_baseline(params, phi, cons, threads):
+    import numpy as np
+
+    from oracle import Oracle
+
+    o = Oracle(params, threads)
+    o.set_state(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+    o.set_step(params.elastep0 + 1)  # skip the zeroth-order solve: time a regular step
+    t0 = time.perf_counter()
+    o.step(1)
+    t1 = time.perf_counter()
+    o.close()
+    return t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--probe", default="z", choices=["x_r2c", "y", "z", "x_c2r", "chem", "polar",
+                                                      "elener", "ks_elastic"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build keeps the whole grid on a single MI355X "
+              "and has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from meumapps_ss_amd.configs import initial_fields, workload
+    from meumapps_ss_amd.params import MM_FP64, MM_MIXED
+
+    torch.cuda.set_device(0)
+    w = workload(2, n=args.size)
+    p = w.params()
+    phi, cons = initial_fields(w)
+    n3 = args.size ** 3
+    kinds = {"x_r2c": 0, "y": 1, "z": 2, "x_c2r": 3, "chem": 4, "polar": 5, "elener": 6, "ks_elastic": 7}
+    kind = kinds[args.probe]
+
+    r64 = run_gpu(p, phi, cons, MM_FP64, args.steps, args.warmup, kind)
+    cups64 = n3 * args.steps / r64["seconds"]
+    n_el = [r["n_elastic"] for r in r64["rows"]]
+    n_avg = sum(n_el) / len(n_el)
+    out = {
+        "metric": "cell-updates/s at 256^3 3-variant: FP64 vs mixed FP32 at 1e-4; % HBM peak",
+        "value": cups64,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r64["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (ran_2 uniform noise, seed 7)",
+        "config": {"workload": f"{args.size}^3 c + 3 tetragonal variants, semi-implicit spectral step, FP64",
+                   "grid": args.size, "variants": 3, "compositions": 1, "dt": 0.005, "seed": 7,
+                   "elastic_iterations_per_step": n_avg, "parallelism": "single GPU"},
+    }
+    if r64["probe_launches"]:
+        avg_ms = r64["probe_ms"] / r64["probe_launches"]
+        bytes_per_launch = r64["probe_bytes"] / r64["probe_launches"]
+        ach = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": args.probe,
+                           "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
+                           "launches": r64["probe_launches"]}
+    bpc = step_bytes_per_cell(3, 1, n_avg, 8)
+    out["step_roofline"] = {"bytes_per_cell_update": bpc,
+                            "achieved_GBs": bpc * cups64 / 1e9,
+                            "frac": bpc * cups64 / 1e9 / HBM_PEAK_GBS}
+    if not args.no_mixed:
+        r32 = run_gpu(p, phi, cons, MM_MIXED, args.steps, args.warmup, kind)
+        cups32 = n3 * args.steps / r32["seconds"]
+        l2 = {}
+        for k in ("phi", "cons"):
+            a, b = r64["state"][k], r32["state"][k]
+            l2[k] = float(np.linalg.norm(b - a) / np.linalg.norm(a))
+            l2[k + "_maxabs"] = float(np.abs(b - a).max())
+        bpc32 = step_bytes_per_cell(3, 1, n_avg, 4)
+        out["mixed"] = {"value": cups32, "speedup_vs_fp64": cups32 / cups64, "rel_l2_vs_fp64": l2,
+                        "ms_per_step": 1e3 * r32["seconds"] / args.steps,
+                        "step_roofline_frac": bpc32 * cups32 / 1e9 / HBM_PEAK_GBS}
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        secs = cpu_baseline(p, phi, cons, threads)
+        out["cpu_baseline"] = {"value": n3 / secs, "unit": "cell-updates/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"1 step of the {args.size}^3 3-variant workload (after the "
+                                         f"elastep0 step) by the CPU binary64 oracle, {threads} OpenMP threads",
+                               "seconds": secs}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -139,6 +139,23 @@ int mm_field(mm_ctx *ctx, int32_t which, void **dev_ptr, int32_t *elem_size,
 /* Copy one SoA field group to the host as binary64 (converts in MIXED). */
 int mm_field_to_host(mm_ctx *ctx, int32_t which, double *host);
 
+/* Live kernel timing (no reference counterpart; the reference only times
+ * eldis with MPI_Wtime, Multi_MEUMAPPS.f:861-871).  mm_probe selects one
+ * kernel class; every launch of it inside mm_step is bracketed by HIP events
+ * on the context's stream.  mm_probe_read returns the accumulated kernel
+ * time, launch count and algorithmic bytes (reads + writes) of those launches
+ * and resets the counters.  kind = -1 switches timing off. */
+#define MM_PROBE_X_R2C 0
+#define MM_PROBE_Y 1
+#define MM_PROBE_Z 2
+#define MM_PROBE_X_C2R 3
+#define MM_PROBE_CHEM 4
+#define MM_PROBE_POLAR 5
+#define MM_PROBE_ELENER 6
+#define MM_PROBE_KS_ELASTIC 7
+int mm_probe(mm_ctx *ctx, int32_t kind);
+int mm_probe_read(mm_ctx *ctx, double *ms_total, int64_t *launches, double *bytes_total);
+
 /* ---- FFT interface: replaces f_trans / inv_trans over P3DFFT (ModFunc.f:15-40) ----
  * Spectrum layout (device): element (kx, ky, kz), kx in [0, Nx/2],
  * at index ((kz * (Nx/2+1)) + kx) * Ny + ky, interleaved complex.           */

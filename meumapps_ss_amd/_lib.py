@@ -54,6 +54,8 @@ def load() -> ctypes.CDLL:
         "mm_ic_uniform": (c_int32, [c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, c_double,
                                     D, D, D]),
         "mm_errstr": (ctypes.c_char_p, [c_int32]),
+        "mm_probe": (c_int32, [c_void_p, c_int32]),
+        "mm_probe_read": (c_int32, [c_void_p, POINTER(c_double), POINTER(ctypes.c_int64), POINTER(c_double)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -72,5 +74,5 @@ EXPORTED = [
     "mm_create", "mm_destroy", "mm_upload", "mm_download", "mm_step", "mm_get_step",
     "mm_set_step", "mm_field", "mm_field_to_host", "mm_fft_create", "mm_fft_destroy",
     "mm_fft_forward", "mm_fft_inverse", "mm_fft_spectrum_elems", "mm_ran2", "mm_ic_uniform",
-    "mm_errstr",
+    "mm_errstr", "mm_probe", "mm_probe_read",
 ]

@@ -3,14 +3,31 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 namespace mm {
 
 // Supported line lengths: y/z axes take N, the x pass takes Nx/2.
 bool fft_length_ok(int n);  // 8..512, 2^a 3^b (a >= 2 for x/2 to stay even-free of 2-length)
 
+// Live timing of one kernel class with HIP events recorded on the stream the
+// kernel is launched on.  Classes: the MM_PROBE_* ids of meumapps_ss.h.
+struct Probe {
+  int kind = -1;
+  std::vector<hipEvent_t> ev;  // begin/end pairs
+  int used = 0;
+  double bytes = 0.0, ms = 0.0;
+  long long launches = 0;
+  void begin(int k, hipStream_t s);
+  void end(int k, hipStream_t s, double b);
+  int collect();  // after the stream is idle: accumulate elapsed time
+  void reset();
+  void destroy();
+};
+
 struct Fft3 {
   int nx = 0, ny = 0, nz = 0, H = 0, prec = 0;
+  Probe* probe = nullptr;
   void* twLx = nullptr;  // exp(-2 pi i e/(Nx/2)), e < Nx/2
   void* twNx = nullptr;  // exp(-2 pi i e/Nx),     e < Nx
   void* twNy = nullptr;

@@ -91,6 +91,41 @@ void Fft3::destroy() {
   twLx = twNx = twNy = twNz = nullptr;
 }
 
+// ---------------------------------------------------------------- probe
+void Probe::begin(int k, hipStream_t s) {
+  if (k != kind) return;
+  if ((int)ev.size() < 2 * (used + 1)) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    ev.push_back(a);
+    ev.push_back(b);
+  }
+  (void)hipEventRecord(ev[2 * used], s);
+}
+void Probe::end(int k, hipStream_t s, double b) {
+  if (k != kind || (int)ev.size() < 2 * (used + 1)) return;
+  (void)hipEventRecord(ev[2 * used + 1], s);
+  used += 1;
+  bytes += b;
+  launches += 1;
+}
+int Probe::collect() {
+  for (int i = 0; i < used; ++i) {
+    float t = 0.f;
+    MM_HIP(hipEventSynchronize(ev[2 * i + 1]));
+    MM_HIP(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+    ms += (double)t;
+  }
+  used = 0;
+  return 0;
+}
+void Probe::reset() { used = 0; bytes = 0.0; ms = 0.0; launches = 0; }
+void Probe::destroy() {
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  ev.clear();
+  reset();
+}
+
 // ------------------------------------------------------------- dispatch
 #define MM_XLENS(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 #define MM_NLENS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
@@ -142,12 +177,14 @@ template <typename T>
 int Fft3::x_r2c(const T* const* in, void* const* out, int nf, hipStream_t s, T scale) const {
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
   Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
+  if (probe) probe->begin(MM_PROBE_X_R2C, s);
   switch (nx / 2) {
 #define X(L_) case L_: go_x_r2c<T, L_>(b, nf, twLx, twNx, ny, nz, scale, s); break;
     MM_XLENS(X)
 #undef X
     default: return MM_ERR_ARG;
   }
+  if (probe) probe->end(MM_PROBE_X_R2C, s, (double)nf * (nreal() * sizeof(T) + ncplx() * 2 * sizeof(T)));
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -156,12 +193,14 @@ template <typename T>
 int Fft3::x_c2r(void* const* in, T* const* out, int nf, hipStream_t s) const {
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
   Batch<T> b = make_batch<T>((const void* const*)in, (void* const*)out, nf);
+  if (probe) probe->begin(MM_PROBE_X_C2R, s);
   switch (nx / 2) {
 #define X(L_) case L_: go_x_c2r<T, L_>(b, nf, twLx, twNx, ny, nz, s); break;
     MM_XLENS(X)
 #undef X
     default: return MM_ERR_ARG;
   }
+  if (probe) probe->end(MM_PROBE_X_C2R, s, (double)nf * (nreal() * sizeof(T) + ncplx() * 2 * sizeof(T)));
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -171,12 +210,14 @@ int Fft3::y_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
   Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
   int nlines = nz * H;
+  if (probe) probe->begin(MM_PROBE_Y, s);
   switch (ny) {
 #define X(L_) case L_: go_y<T, L_>(b, nf, twNy, nlines, dir, s); break;
     MM_NLENS(X)
 #undef X
     default: return MM_ERR_ARG;
   }
+  if (probe) probe->end(MM_PROBE_Y, s, (double)nf * 2.0 * ncplx() * 2 * sizeof(T));
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -186,12 +227,14 @@ int Fft3::z_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
   Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
   int ncols = H * ny;
+  if (probe) probe->begin(MM_PROBE_Z, s);
   switch (nz) {
 #define X(L_) case L_: go_z<T, L_>(b, nf, twNz, ncols, dir, s); break;
     MM_NLENS(X)
 #undef X
     default: return MM_ERR_ARG;
   }
+  if (probe) probe->end(MM_PROBE_Z, s, (double)nf * 2.0 * ncplx() * 2 * sizeof(T));
   MM_HIP(hipGetLastError());
   return 0;
 }

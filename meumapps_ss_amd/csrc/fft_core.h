@@ -17,6 +17,8 @@
 
 namespace mm {
 
+constexpr int kMaxBatch = 16;  // fields per batched FFT launch
+
 template <typename T> struct C2T;
 template <> struct C2T<double> { using type = double2; };
 template <> struct C2T<float> { using type = float2; };

@@ -20,8 +20,6 @@
 
 namespace mm {
 
-constexpr int kMaxBatch = 16;
-
 template <typename T> struct Batch {
   const void* in[kMaxBatch];
   void* out[kMaxBatch];

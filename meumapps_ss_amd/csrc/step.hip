@@ -1,12 +1,1302 @@
-// step.hip -- placeholder until the step pipeline lands.
-#include "mm_common.h"
+// step.hip -- device state, kernels and host orchestration of one
+// MEUMAPPS-SS semi-implicit spectral step (Multi_MEUMAPPS.f:548-1106), and
+// the mm_* whole-step entry points of include/meumapps_ss.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
 
-extern "C" int mm_create(const mm_params*, int, mm_ctx** out) { if (out) *out = nullptr; return MM_ERR_ARG; }
-extern "C" void mm_destroy(mm_ctx*) {}
-extern "C" int mm_upload(mm_ctx*, const double*, const double*, const double*, const double*, const double*) { return MM_ERR_ARG; }
-extern "C" int mm_download(mm_ctx*, double*, double*, double*, double*, double*) { return MM_ERR_ARG; }
-extern "C" int mm_step(mm_ctx*, int32_t, mm_diag*) { return MM_ERR_ARG; }
-extern "C" int mm_get_step(const mm_ctx*) { return 0; }
-extern "C" int mm_set_step(mm_ctx*, int32_t) { return MM_ERR_ARG; }
-extern "C" int mm_field(mm_ctx*, int32_t, void**, int32_t*, int32_t*) { return MM_ERR_ARG; }
-extern "C" int mm_field_to_host(mm_ctx*, int32_t, double*) { return MM_ERR_ARG; }
+#include "fft3d.h"
+#include "mm_common.h"
+#include "step_kernels.h"
+
+namespace mm {
+
+// ============================================================== kernels
+
+__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ part, int nblk, int K,
+                                                     FinalizeOps ops, double* __restrict__ out) {
+  __shared__ double sh[kWaves];
+  for (int k = 0; k < K; ++k) {
+    const int op = ops.op[k];
+    double acc = op == 0 ? 0.0 : (op == 1 ? HUGE_VAL : -HUGE_VAL);
+    for (int b = threadIdx.x; b < nblk; b += kBlock) {
+      const double x = part[(size_t)k * nblk + b];
+      acc = op == 0 ? acc + x : (op == 1 ? (x < acc ? x : acc) : (x > acc ? x : acc));
+    }
+    double r = op == 0 ? block_reduce<0>(acc, sh) : (op == 1 ? block_reduce<1>(acc, sh) : block_reduce<2>(acc, sh));
+    if (threadIdx.x == 0) out[k] = r;
+  }
+}
+
+#define GRID_STRIDE(i, n) \
+  for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < (n); i += (size_t)gridDim.x * kBlock)
+
+// -------- Gaussian elimination, partial pivoting, N compile-time (KKS Zintf)
+template <int N>
+__device__ __forceinline__ bool solve_pivot(double (&A)[N][N], double (&b)[N]) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    int p = k;
+    double amax = fabs(A[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double a = fabs(A[i][k]);
+      if (a > amax) { amax = a; p = i; }
+    }
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      if (i == p) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) { double t = A[k][j]; A[k][j] = A[i][j]; A[i][j] = t; }
+        double t = b[k]; b[k] = b[i]; b[i] = t;
+      }
+    }
+    if (!(amax > 0.0)) ok = false;
+    const double inv = 1.0 / A[k][k];
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double l = A[i][k] * inv;
+#pragma unroll
+      for (int j = k + 1; j < N; ++j) A[i][j] -= l * A[k][j];
+      b[i] -= l * b[k];
+    }
+  }
+#pragma unroll
+  for (int k = N - 1; k >= 0; --k) {
+    double s = b[k];
+#pragma unroll
+    for (int j = k + 1; j < N; ++j) s -= A[k][j] * b[j];
+    b[k] = s / A[k][k];
+  }
+  return ok;
+}
+
+// -------- 6x6 solve without pivoting: the guarded mixture of the elastic
+// iteration (ModElastic.f:262-268) is a nonnegative combination of SPD
+// compliances once phi is in [0,1]
+__device__ __forceinline__ bool solve6(double (&A)[6][6], double (&b)[6]) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (!(A[k][k] > 0.0)) ok = false;
+    const double inv = 1.0 / A[k][k];
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      const double l = A[i][k] * inv;
+#pragma unroll
+      for (int j = k + 1; j < 6; ++j) A[i][j] -= l * A[k][j];
+      b[i] -= l * b[k];
+    }
+  }
+#pragma unroll
+  for (int k = 5; k >= 0; --k) {
+    double s = b[k];
+#pragma unroll
+    for (int j = k + 1; j < 6; ++j) s -= A[k][j] * b[j];
+    b[k] = s / A[k][k];
+  }
+  return ok;
+}
+
+__device__ __forceinline__ double pfun(double f) {  // p(phi), Multi_MEUMAPPS.f:653
+  const double fs = f * f;
+  return fs * f * (6.0 * fs - 15.0 * f + 10.0);
+}
+__device__ __forceinline__ double ppr(double f) {  // p'(phi), :656
+  const double fs = f * f;
+  return 30.0 * fs * (fs - 2.0 * f + 1.0);
+}
+
+// ---------------------------------------------------------------- chemistry
+// Multi_MEUMAPPS.f:634-845: p, p', Pph, Gph, KKS interface compositions,
+// Gibbs energies, chemical potential, driving forces f_grad, fluxes F and
+// the f_diff correction.  Partial sums: f_int, fch, fw.
+template <typename T, int V, int NPH, int E>
+__global__ void __launch_bounds__(kBlock)
+k_chem(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ gphi,
+       const T* __restrict__ kdel, const T* __restrict__ cons, T* __restrict__ fdiff,
+       T* __restrict__ flux, double* __restrict__ part, int* __restrict__ err, size_t n3) {
+  constexpr int Z = NPH * E;
+  __shared__ double sh[kWaves];
+  double s_fint = 0.0, s_fch = 0.0, s_fw = 0.0;
+  bool bad = false;
+  GRID_STRIDE(pt, n3) {
+    double f[V], gx[V], gy[V], gz[V], kd[V], c[E];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      f[v] = ld(phi, v * n3 + pt);
+      gx[v] = ld(gphi, (3 * v + 0) * n3 + pt);
+      gy[v] = ld(gphi, (3 * v + 1) * n3 + pt);
+      gz[v] = ld(gphi, (3 * v + 2) * n3 + pt);
+      kd[v] = ld(kdel, v * n3 + pt);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) c[e] = P->cbar[e] + ld(cons, e * n3 + pt);
+    double Ppr[V], Pph[NPH + 1], Gph[NPH + 1];
+#pragma unroll
+    for (int q = 0; q <= NPH; ++q) { Pph[q] = 0.0; Gph[q] = 0.0; }
+#pragma unroll
+    for (int q = 2; q <= NPH; ++q)
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if (P->phase_of[v] == q) {
+          const double fi = f[v], fs = fi * fi;
+          const double pl = fs * fi * (6.0 * fs - 15.0 * fi + 10.0);
+          Ppr[v] = 30.0 * fs * (fs - 2.0 * fi + 1.0);
+          Pph[1] += pl;
+          Pph[q] += pl;
+          Gph[q] += fs * ((1.0 - fi) * (1.0 - fi));
+#pragma unroll
+          for (int jv = 0; jv < V; ++jv)
+            if (jv != v) Gph[1] += fs * (f[jv] * f[jv]);
+        }
+    Pph[1] = 1.0 - Pph[1];
+    // Zintf x = Zr (constant rows :426-448, mixture rows :677-688)
+    double A[Z][Z], b[Z];
+#pragma unroll
+    for (int i = 0; i < Z; ++i) {
+#pragma unroll
+      for (int j = 0; j < Z; ++j) A[i][j] = P->zc[i * 16 + j];
+      b[i] = P->zrc[i];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i0 = Z - E + e;
+#pragma unroll
+      for (int ph = 0; ph < NPH; ++ph) A[i0][NPH * e + ph] = Pph[ph + 1];
+      b[i0] = c[e];
+    }
+    if (!solve_pivot<Z>(A, b)) bad = true;
+    // b[NPH*e + ph] = Xintf(e+1, ph+1)
+    const double* gp = P->gp;
+#define GP(j, ph) gp[((ph) - 1) * MM_GP_STRIDE + ((j) - 1)]
+    double Gf[NPH + 1], Mu[E];
+#pragma unroll
+    for (int ph = 1; ph <= NPH; ++ph) {
+      double g = 0.0;
+#pragma unroll
+      for (int e = 1; e <= E; ++e) {
+        const double dx = b[NPH * (e - 1) + ph - 1] - GP(e + E + 1, ph);
+        g += GP(e, ph) * (dx * dx);
+      }
+      Gf[ph] = g + GP(E + 1, ph);
+    }
+    Gf[0] = 0.0;
+#pragma unroll
+    for (int e = 1; e <= E; ++e) Mu[e - 1] = 2.0 * GP(e, 1) * (b[NPH * (e - 1)] - GP(e + E + 1, 1));
+    double fch = Pph[1] * Gf[1];
+    double fw = P->well[0] * Gph[1];
+    double Fx[E], Fy[E], Fz[E], fg[V];
+#pragma unroll
+    for (int e = 0; e < E; ++e) { Fx[e] = 0.0; Fy[e] = 0.0; Fz[e] = 0.0; }
+#pragma unroll
+    for (int q = 2; q <= NPH; ++q) {
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if (P->phase_of[v] == q) {
+          const double* g = P->gc + v * 9;
+          const double term = g[0] * (gx[v] * gx[v]) + g[4] * (gy[v] * gy[v]) + g[8] * (gz[v] * gz[v]) +
+                              (g[3] + g[1]) * gx[v] * gy[v] + (g[6] + g[2]) * gx[v] * gz[v] +
+                              (g[7] + g[5]) * gy[v] * gz[v];
+          s_fint += 0.5 * term;
+          const double fi = f[v];
+          double termch = Gf[q] - Gf[1];
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const double xd = b[NPH * e] - b[NPH * e + q - 1];
+            termch = termch + Mu[e] * xd;
+            const double tprc = Ppr[v] * xd;
+            Fx[e] += tprc * gx[v];
+            Fy[e] += tprc * gy[v];
+            Fz[e] += tprc * gz[v];
+          }
+          double wg = 2.0 * P->well[q - 1] * fi * (1.0 - fi) * (1.0 - 2.0 * fi);
+#pragma unroll
+          for (int jv = 0; jv < V; ++jv)
+            if (jv != v) wg = wg + 4.0 * P->well[0] * fi * (f[jv] * f[jv]);
+          fg[v] = termch * Ppr[v] + wg;
+        }
+      fch = fch + Pph[q] * Gf[q];
+      fw = fw + P->well[q - 1] * Gph[q];
+    }
+#undef GP
+    s_fch += fch;
+    s_fw += fw;
+    // f_diff correction (:824-841)
+#pragma unroll
+    for (int jv = 0; jv < V; ++jv) {
+      double t11 = 0.0;
+      int nt = 1;
+      if (f[jv] > 0.0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if (v != jv && f[v] > 0.0) { t11 = t11 + fg[v] - kd[v]; nt += 1; }
+      }
+      st(fdiff, jv * n3 + pt, fg[jv] - t11 / (double)nt);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      st(flux, (3 * e + 0) * n3 + pt, Fx[e]);
+      st(flux, (3 * e + 1) * n3 + pt, Fy[e]);
+      st(flux, (3 * e + 2) * n3 + pt, Fz[e]);
+    }
+  }
+  block_store<0>(s_fint, part, 0, sh);
+  block_store<0>(s_fch, part, 1, sh);
+  block_store<0>(s_fw, part, 2, sh);
+  if (bad) *err = MM_ERR_ZSOLVE;
+}
+
+// p(phi) fields for the zeroth-order elastic solve (ModElastic.f:105-112)
+template <typename T, int V>
+__global__ void __launch_bounds__(kBlock) k_ppoly(const T* __restrict__ phi, T* __restrict__ out, size_t n3) {
+  GRID_STRIDE(pt, n3) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) st(out, v * n3 + pt, pfun(ld(phi, v * n3 + pt)));
+  }
+}
+
+// compliance mixture  sum_v P_v S_ph(v) [+ (1 - sum P) S_1]  (ModElastic.f:252-268, :583-597)
+template <int V, int NPH>
+__device__ __forceinline__ void compliance_mix(const DevParams* __restrict__ P, const double (&Pv)[V],
+                                               double ps, bool matrix, double (&M)[6][6]) {
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) M[a][c] = 0.0;
+#pragma unroll
+  for (int q = 2; q <= NPH; ++q)
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (P->phase_of[v] == q) {
+        const double* S = P->sela + (q - 1) * 36;
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) M[a][c] = M[a][c] + Pv[v] * S[c * 6 + a];
+      }
+  if (matrix) {
+    const double w = 1.0 - ps;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) M[a][c] = M[a][c] + w * P->sela[c * 6 + a];
+  }
+}
+
+// ------------------------------------------------- stress polarisation x_ij
+// ModElastic.f:248-327: C(x) = (sum P S)^-1, C' = CEla(:,:,2) - C(x),
+// x = cfac C' (eps + e_mean) + C(x) sum_v P_v eps0_v  with engineering
+// shears, evaluated as  cfac CEla2 b + C(x) (a - cfac b)  by one 6x6 solve.
+template <typename T, int V, int NPH>
+__global__ void __launch_bounds__(kBlock)
+k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ strain,
+        double cfac, T* __restrict__ xpol, int* __restrict__ err, size_t n3) {
+  bool bad = false;
+  GRID_STRIDE(pt, n3) {
+    double Pv[V], ps = 0.0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) { Pv[v] = pfun(ld(phi, v * n3 + pt)); ps = ps + Pv[v]; }
+    double M[6][6];
+    compliance_mix<V, NPH>(P, Pv, ps, ps < 1.0, M);
+    double bv[6], r[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const double w = m < 3 ? 1.0 : 2.0;
+      bv[m] = ld(strain, m * n3 + pt) + w * P->e_mean[m];
+      double a = 0.0;
+#pragma unroll
+      for (int v = 0; v < V; ++v) a += Pv[v] * P->tp[v * 6 + m];
+      r[m] = w * a - cfac * bv[m];
+    }
+    if (!solve6(M, r)) bad = true;
+    const double* C2m = P->cela + 36;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      double s = 0.0;
+#pragma unroll
+      for (int n = 0; n < 6; ++n) s += C2m[n * 6 + m] * bv[n];
+      st(xpol, m * n3 + pt, cfac * s + r[m]);
+    }
+  }
+  if (bad) *err = MM_ERR_CINV_ITER;
+}
+
+// --------------------------------------------------------------- elener
+// ModElastic.f:493-691 fused with f_diff += el_grad (Multi_MEUMAPPS.f:880).
+template <typename T, int V, int NPH>
+__global__ void __launch_bounds__(kBlock)
+k_elener(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ strain,
+         T* __restrict__ el_en, T* __restrict__ elgrad, T* __restrict__ fdiff,
+         double* __restrict__ part, int* __restrict__ err, size_t n3) {
+  __shared__ double sh[kWaves];
+  double s_el = 0.0;
+  bool bad = false;
+  GRID_STRIDE(pt, n3) {
+    double f[V], Pv[V], ps = 0.0, te[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      f[v] = ld(phi, v * n3 + pt);
+      Pv[v] = pfun(f[v]);
+      ps = ps + Pv[v];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) te[m] = te[m] + Pv[v] * P->tp[v * 6 + m];
+    }
+    double sig[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const double w = m < 3 ? 1.0 : 2.0;
+      sig[m] = ld(strain, m * n3 + pt) + w * P->e_mean[m] - w * te[m];  // eps
+    }
+    // the matrix term is unguarded here (ModElastic.f:597): with sum P > 1 the
+    // mixture can be indefinite, so this solve pivots
+    double M[6][6];
+    compliance_mix<V, NPH>(P, Pv, ps, true, M);
+    if (!solve_pivot<6>(M, sig)) bad = true;  // sig = C(x) eps
+    // el_mat = 1/2 sig . S_1 sig
+    double el_mat = 0.0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      double em = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) em = em + P->sela[c * 6 + a] * sig[c];
+      el_mat = el_mat + sig[a] * em;
+    }
+    el_mat = 0.5 * el_mat;
+    double een = (1.0 - ps) * el_mat;
+#pragma unroll
+    for (int q = 2; q <= NPH; ++q) {
+      double el_ppt = 0.0;
+      const double* S = P->sela + (q - 1) * 36;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+        double ep = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ep = ep + S[c * 6 + a] * sig[c];
+        el_ppt = el_ppt + sig[a] * ep;
+      }
+      el_ppt = 0.5 * el_ppt;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if (P->phase_of[v] == q) {
+          double eg = 0.0;
+          if (P->tsum_ok[v]) {
+            een = een + el_ppt * Pv[v];
+            const double* t = P->tp + v * 6;
+            const double d = (sig[0] * t[0] + sig[1] * t[1] + sig[2] * t[2] +
+                              2.0 * (sig[3] * t[3] + sig[4] * t[4] + sig[5] * t[5]));
+            const double pp = ppr(f[v]);
+            eg = pp * (el_mat - el_ppt) - d * pp;
+          }
+          st(elgrad, v * n3 + pt, eg);
+          st(fdiff, v * n3 + pt, ld(fdiff, v * n3 + pt) + eg);
+        }
+    }
+    st(el_en, pt, een);
+    s_el += een;
+  }
+  block_store<0>(s_el, part, 0, sh);
+  if (bad) *err = MM_ERR_CINV_ENER;
+}
+
+// ------------------------------------------------- clamp + check loop sums
+// Multi_MEUMAPPS.f:1016-1043: clamp phi to [0,1]; sum c, min/max c, max phi,
+// sum of phi > 0.  Slots: 0..E-1 sum c (stored values), E min, E+1 max,
+// E+2 max phi, E+3 solid sum.
+template <typename T, int V, int E>
+__global__ void __launch_bounds__(kBlock)
+k_clamp(const DevParams* __restrict__ P, T* __restrict__ phi, const T* __restrict__ cons,
+        double* __restrict__ part, size_t n3) {
+  __shared__ double sh[kWaves];
+  double cs[E], cmin = HUGE_VAL, cmax = -HUGE_VAL, pmax = -HUGE_VAL, sf = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) cs[e] = 0.0;
+  GRID_STRIDE(pt, n3) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double raw = ld(cons, e * n3 + pt);
+      const double c = P->cbar[e] + raw;
+      if (c < cmin) cmin = c;
+      if (c > cmax) cmax = c;
+      cs[e] += raw;
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double f = ld(phi, v * n3 + pt);
+      if (f < 0.0) f = 0.0;
+      if (f > 1.0) f = 1.0;
+      st(phi, v * n3 + pt, f);
+      f = (double)(T)f;
+      if (f > pmax) pmax = f;
+      if (f > 0.0) sf = sf + f;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) block_store<0>(cs[e], part, e, sh);
+  block_store<1>(cmin, part, E, sh);
+  block_store<2>(cmax, part, E + 1, sh);
+  block_store<2>(pmax, part, E + 2, sh);
+  block_store<0>(sf, part, E + 3, sh);
+}
+
+// ============================================================ k-space kernels
+
+// gradient block (Multi_MEUMAPPS.f:581-605): out[4j+d] = phi_hat * i kf_d (d<3),
+// out[4j+3] = phi_hat * sum G kk.  blockIdx.y = j, variant v0 + j.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_ks_grad(const DevParams* __restrict__ P, const C2<T>* __restrict__ sphi, C2<T>* __restrict__ work,
+          int v0, size_t nc) {
+  const int j = blockIdx.y, v = v0 + j;
+  const C2<T>* in = sphi + (size_t)v * nc;
+  C2<T>* o = work + (size_t)4 * j * nc;
+  GRID_STRIDE(i, nc) {
+    const KVec k = kvec(P, i);
+    const C2<T> s = in[i];
+    const double sr = (double)s.x, si = (double)s.y;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o[(size_t)d * nc + i] = mk<T>((T)(-si * k.kf[d]), (T)(sr * k.kf[d]));
+    const double g = kgk(P, v, k);
+    o[(size_t)3 * nc + i] = mk<T>((T)(sr * g), (T)(si * g));
+  }
+}
+
+// zeroth-order displacement at step == elastep0 (ModElastic.f:115-236):
+// R_i = i sum_j kf_j sum_v Re(P_hat_v) sig0_v(ij) (sum_0 is real*8, so only the
+// real part of P_hat survives), u = Q^-1 R, u(k=0) = 0.
+template <typename T, int V>
+__global__ void __launch_bounds__(kBlock)
+k_ks_zeroth(const DevParams* __restrict__ P, const C2<T>* __restrict__ phat, C2<T>* __restrict__ uhat,
+            C2<T>* __restrict__ work, size_t nc) {
+  C2<T>* outs[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) outs[m] = work + (size_t)m * nc;
+  const int vm[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+  GRID_STRIDE(i, nc) {
+    const KVec k = kvec(P, i);
+    Cplx3 u;
+    if (k.kx == 0 && k.ky == 0 && k.kz == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { u.re[c] = 0.0; u.im[c] = 0.0; }
+    } else {
+      double pre[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) pre[v] = (double)phat[(size_t)v * nc + i].x;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double ri = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          double s0 = 0.0;
+#pragma unroll
+          for (int v = 0; v < V; ++v) s0 += pre[v] * P->sig0[v * 6 + vm[a][b]];
+          ri += s0 * k.kf[b];
+        }
+        u.re[a] = 0.0;
+        u.im[a] = ri;
+      }
+      acoustic_solve(P, k, u);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) uhat[(size_t)c * nc + i] = mk<T>((T)u.re[c], (T)u.im[c]);
+    store_strain<T>(outs, i, k, u);
+  }
+}
+
+// elastic iteration k-space step (ModElastic.f:340-394): R = kf . x_hat,
+// u = Q^-1 R (u(0) = 0), strain spectra, and Parseval forms of the
+// convergence sums of :413-430 over the half spectrum.  work[0..5] holds
+// x_hat on entry and the strain spectra on exit; uhat holds u_old / u.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_ks_elastic(const DevParams* __restrict__ P, C2<T>* __restrict__ work, C2<T>* __restrict__ uhat,
+             double* __restrict__ part, size_t nc) {
+  __shared__ double sh[kWaves];
+  C2<T>* outs[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) outs[m] = work + (size_t)m * nc;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  GRID_STRIDE(i, nc) {
+    const KVec k = kvec(P, i);
+    double xr[6], xi[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const C2<T> x = outs[m][i];
+      xr[m] = (double)x.x;
+      xi[m] = (double)x.y;
+    }
+    Cplx3 u;
+    if (k.kx == 0 && k.ky == 0 && k.kz == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { u.re[c] = 0.0; u.im[c] = 0.0; }
+    } else {
+      // R(1) = x1 kf1 + x6 kf2 + x5 kf3 ; R(2) = x6 kf1 + x2 kf2 + x4 kf3 ; R(3) = x5 kf1 + x4 kf2 + x3 kf3
+      const int vm[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double rr = 0.0, ri = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          rr += -xi[vm[a][b]] * k.kf[b];
+          ri += xr[vm[a][b]] * k.kf[b];
+        }
+        u.re[a] = rr;
+        u.im[a] = ri;
+      }
+      acoustic_solve(P, k, u);
+    }
+    const double w = (k.kx == 0 || 2 * k.kx == P->nx) ? 1.0 : 2.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const C2<T> uo = uhat[(size_t)c * nc + i];
+      const double dr = u.re[c] - (double)uo.x, di = u.im[c] - (double)uo.y;
+      acc[2 * c] += w * (dr * dr + di * di);
+      acc[2 * c + 1] += w * ((double)uo.x * (double)uo.x + (double)uo.y * (double)uo.y);
+      uhat[(size_t)c * nc + i] = mk<T>((T)u.re[c], (T)u.im[c]);
+    }
+    store_strain<T>(outs, i, k, u);
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) block_store<0>(acc[m], part, m, sh);
+}
+
+// Cahn-Hilliard update (Multi_MEUMAPPS.f:924-943); blockIdx.y = e,
+// work[4e] = c_hat (in/out), work[4e+1..3] = F_hat.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_ks_ch(const DevParams* __restrict__ P, C2<T>* __restrict__ work, size_t nc) {
+  const int e = blockIdx.y;
+  C2<T>* ch = work + (size_t)(4 * e) * nc;
+  const C2<T>* fx = ch + nc;
+  const C2<T>* fy = ch + 2 * nc;
+  const C2<T>* fz = ch + 3 * nc;
+  const double dtm = P->dt * P->cmobs[e];
+  GRID_STRIDE(i, nc) {
+    const KVec k = kvec(P, i);
+    const double ksq = k.kq[0] + k.kq[1] + k.kq[2];
+    const C2<T> a = fx[i], b = fy[i], c = fz[i];
+    // term1 = i (kf1 Fx + kf2 Fy + kf3 Fz)
+    const double tr = -((double)a.y * k.kf[0] + (double)b.y * k.kf[1] + (double)c.y * k.kf[2]);
+    const double ti = (double)a.x * k.kf[0] + (double)b.x * k.kf[1] + (double)c.x * k.kf[2];
+    const double den = 1.0 - dtm * ksq;
+    const C2<T> s = ch[i];
+    ch[i] = mk<T>((T)(((double)s.x + dtm * tr) / den), (T)(((double)s.y + dtm * ti) / den));
+  }
+}
+
+// Allen-Cahn update (Multi_MEUMAPPS.f:966-991), noise term zero;
+// blockIdx.y = v, sphi[v] <- (phi_hat - dt L f_hat) / (1 - dt L sum G kk).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_ks_ac(const DevParams* __restrict__ P, C2<T>* __restrict__ sphi, const C2<T>* __restrict__ fhat, size_t nc) {
+  const int v = blockIdx.y;
+  C2<T>* ph = sphi + (size_t)v * nc;
+  const C2<T>* fh = fhat + (size_t)v * nc;
+  const double dtl = P->dt * P->bmob;
+  GRID_STRIDE(i, nc) {
+    const KVec k = kvec(P, i);
+    const double den = 1.0 - dtl * kgk(P, v, k);
+    const C2<T> s = ph[i], f = fh[i];
+    ph[i] = mk<T>((T)(((double)s.x - dtl * (double)f.x) / den), (T)(((double)s.y - dtl * (double)f.y) / den));
+  }
+}
+
+// out = i kf_d in   (displacement gradients for the dump records)
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_ks_deriv(const DevParams* __restrict__ P, const C2<T>* __restrict__ in, C2<T>* __restrict__ out, int d,
+           size_t nc) {
+  GRID_STRIDE(i, nc) {
+    const KVec k = kvec(P, i);
+    const C2<T> s = in[i];
+    out[i] = mk<T>((T)(-(double)s.y * k.kf[d]), (T)((double)s.x * k.kf[d]));
+  }
+}
+
+// ============================================================ dispatch table
+template <typename T> struct Kern {
+  void (*chem)(dim3, hipStream_t, const DevParams*, const T*, const T*, const T*, const T*, T*, T*, double*, int*, size_t) = nullptr;
+  void (*ppoly)(dim3, hipStream_t, const T*, T*, size_t) = nullptr;
+  void (*polar)(dim3, hipStream_t, const DevParams*, const T*, const T*, double, T*, int*, size_t) = nullptr;
+  void (*elener)(dim3, hipStream_t, const DevParams*, const T*, const T*, T*, T*, T*, double*, int*, size_t) = nullptr;
+  void (*clamp)(dim3, hipStream_t, const DevParams*, T*, const T*, double*, size_t) = nullptr;
+  void (*zeroth)(dim3, hipStream_t, const DevParams*, const C2<T>*, C2<T>*, C2<T>*, size_t) = nullptr;
+};
+
+template <typename T, int V, int NPH, int E> struct KernImpl {
+  static void chem(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, const T* c, const T* d,
+                   T* e, T* f, double* p, int* err, size_t n) {
+    hipLaunchKernelGGL((k_chem<T, V, NPH, E>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, f, p, err, n);
+  }
+  static void ppoly(dim3 g, hipStream_t s, const T* a, T* b, size_t n) {
+    hipLaunchKernelGGL((k_ppoly<T, V>), g, dim3(kBlock), 0, s, a, b, n);
+  }
+  static void polar(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, double cf, T* x,
+                    int* err, size_t n) {
+    hipLaunchKernelGGL((k_polar<T, V, NPH>), g, dim3(kBlock), 0, s, P, a, b, cf, x, err, n);
+  }
+  static void elener(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, T* c, T* d, T* e,
+                     double* p, int* err, size_t n) {
+    hipLaunchKernelGGL((k_elener<T, V, NPH>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, p, err, n);
+  }
+  static void clamp(dim3 g, hipStream_t s, const DevParams* P, T* a, const T* b, double* p, size_t n) {
+    hipLaunchKernelGGL((k_clamp<T, V, E>), g, dim3(kBlock), 0, s, P, a, b, p, n);
+  }
+  static void zeroth(dim3 g, hipStream_t s, const DevParams* P, const C2<T>* a, C2<T>* b, C2<T>* c, size_t n) {
+    hipLaunchKernelGGL((k_ks_zeroth<T, V>), g, dim3(kBlock), 0, s, P, a, b, c, n);
+  }
+  static Kern<T> table() {
+    Kern<T> k;
+    k.chem = chem; k.ppoly = ppoly; k.polar = polar; k.elener = elener; k.clamp = clamp; k.zeroth = zeroth;
+    return k;
+  }
+};
+
+// supported (V, NPH, E)
+#define MM_COMBOS(X) \
+  X(1, 2, 1) X(2, 2, 1) X(3, 2, 1) X(4, 2, 1) X(2, 3, 1) X(3, 3, 1) X(4, 3, 1) X(1, 2, 2) X(3, 2, 2)
+
+template <typename T> static bool kern_table(int V, int NPH, int E, Kern<T>& out) {
+#define X(v_, p_, e_) \
+  if (V == v_ && NPH == p_ && E == e_) { out = KernImpl<T, v_, p_, e_>::table(); return true; }
+  MM_COMBOS(X)
+#undef X
+  return false;
+}
+
+}  // namespace mm
+
+// ================================================================= context
+using namespace mm;
+
+struct mm_ctx {
+  mm_params prm{};
+  int prec = 0;
+  int V = 0, E = 0, NPH = 0;
+  size_t n3 = 0, nc = 0;
+  int step = 0;
+  Fft3 fft;
+  hipStream_t s = nullptr;
+  hipEvent_t ev = nullptr;
+  DevParams hp{};
+  DevParams* dp = nullptr;
+  size_t esz = 8;
+  void *phi = nullptr, *cons = nullptr, *gphi = nullptr, *kdel = nullptr, *fdiff = nullptr, *flux = nullptr,
+       *strain = nullptr, *xpol = nullptr, *el_en = nullptr, *elgrad = nullptr;
+  void *sphi = nullptr, *suhat = nullptr, *swork = nullptr;
+  int nwork = 0, nxpol = 0;
+  double* part = nullptr;
+  double* dres = nullptr;
+  double* hres = nullptr;  // pinned
+  int* derr = nullptr;
+  int nblk_pw = 0, nblk_k = 0;
+  double glob_min = 1.0, glob_max = 0.0, phi_max = 0.0, solid_frac = 0.0;
+  Kern<double> k64;
+  Kern<float> k32;
+  Probe probe;
+  int last_niter = 0;
+  double last_cfac = 1.0;
+};
+
+namespace {
+
+constexpr int kMaxPartSlots = 16;
+constexpr int kMaxBlocks = 2048;
+// result slots in dres
+enum { R_FINT = 0, R_FCH, R_FW, R_EL, R_ERR0 /* 6 */, R_CLAMP = 10 /* E+4 */, R_N = 24 };
+
+int blocks_for(size_t n) { return (int)std::min<size_t>((n + kBlock - 1) / kBlock, (size_t)kMaxBlocks); }
+
+void host_params(const mm_params& p, int prec, DevParams& d) {
+  std::memset(&d, 0, sizeof(d));
+  d.nx = p.nx; d.ny = p.ny; d.nz = p.nz; d.H = p.nx / 2 + 1;
+  d.V = p.vartot; d.E = p.numel; d.NPH = p.numph; d.zdim = p.numph * p.numel;
+  int ivn = 0;
+  for (int ph = 2; ph <= p.numph; ++ph) {
+    ivn += p.nvar[ph - 2];
+    for (int v = ivn; v <= ivn + p.nvar[ph - 1] - 1; ++v) d.phase_of[v - 1] = ph;
+  }
+  for (int v = 0; v < p.vartot; ++v) {
+    double s = 0.0;
+    for (int m = 0; m < 6; ++m) s += p.trans_ppt[v * 6 + m];
+    d.tsum_ok[v] = s > (double)1.e-9f;
+  }
+  const double pi = std::atan(1.0) * 4.0;
+  d.unit[0] = (2.0 * pi) / (double)(float)p.nx;
+  d.unit[1] = (2.0 * pi) / (double)(float)p.ny;
+  d.unit[2] = (2.0 * pi) / (double)(float)p.nz;
+  d.dt = p.t_step; d.bmob = p.b_mob;
+  for (int e = 0; e < MM_MAX_EL; ++e) d.cmobs[e] = p.cmobs[e];
+  std::memcpy(d.gc, p.grad_coeff, sizeof(d.gc));
+  std::memcpy(d.tp, p.trans_ppt, sizeof(d.tp));
+  std::memcpy(d.cela, p.cela, sizeof(d.cela));
+  std::memcpy(d.sela, p.sela, sizeof(d.sela));
+  std::memcpy(d.gp, p.gparams, sizeof(d.gp));
+  std::memcpy(d.well, p.wellph, sizeof(d.well));
+  std::memcpy(d.e_mean, p.e_mean, sizeof(d.e_mean));
+  // sig0_v(ij) = sum_m CEla(ij,m,ph(v)) w_m trans_ppt(m,v)
+  for (int v = 0; v < p.vartot; ++v) {
+    const int ph = d.phase_of[v];
+    for (int a = 0; a < 6; ++a) {
+      double s = 0.0;
+      for (int m = 0; m < 6; ++m) s += p.cela[(ph - 1) * 36 + m * 6 + a] * (m < 3 ? 1.0 : 2.0) * p.trans_ppt[v * 6 + m];
+      d.sig0[v * 6 + a] = s;
+    }
+  }
+  // constant rows of Zintf / Zr (Multi_MEUMAPPS.f:430-448)
+  const int E = p.numel, NPH = p.numph;
+  auto GP = [&](int j, int ph) { return p.gparams[(ph - 1) * MM_GP_STRIDE + (j - 1)]; };
+  for (int e = 1; e <= E; ++e) {
+    const int i1 = e + E + 1;
+    for (int ph = 1; ph <= NPH - 1; ++ph) {
+      const int i0 = (NPH - 1) * (e - 1) + ph, j0 = NPH * (e - 1) + ph;
+      d.zc[(i0 - 1) * 16 + (j0 - 1)] = GP(e, ph);
+      d.zc[(i0 - 1) * 16 + j0] = -GP(e, ph + 1);
+      d.zrc[i0 - 1] = GP(e, ph) * GP(i1, ph) - GP(e, ph + 1) * GP(i1, ph + 1);
+    }
+  }
+  (void)prec;
+}
+
+int dalloc(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; return MM_ERR_NOMEM; }
+  return 0;
+}
+
+template <typename T> T* F(void* base, size_t n, int k) { return (T*)base + (size_t)k * n; }
+template <typename T> C2<T>* S(void* base, size_t nc, int k) { return (C2<T>*)base + (size_t)k * nc; }
+
+int finalize(mm_ctx* c, int nblk, int K, std::initializer_list<int> ops, int slot0) {
+  FinalizeOps fo;
+  int i = 0;
+  for (int o : ops) fo.op[i++] = o;
+  for (; i < 8; ++i) fo.op[i] = 0;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->s, c->part, nblk, K, fo, c->dres + slot0);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T> const Kern<T>& kern(const mm_ctx* c);
+template <> const Kern<double>& kern<double>(const mm_ctx* c) { return c->k64; }
+template <> const Kern<float>& kern<float>(const mm_ctx* c) { return c->k32; }
+
+// batched forward FFT of nf real fields (stride n3) into spectra (stride nc)
+template <typename T> int fwd(mm_ctx* c, T* in, size_t in_stride, C2<T>* out, int nf) {
+  const T scale = (T)(1.0 / (double)(float)c->n3);
+  for (int f0 = 0; f0 < nf; f0 += kMaxBatch) {
+    const int g = std::min(kMaxBatch, nf - f0);
+    const T* ip[kMaxBatch];
+    void* op[kMaxBatch];
+    for (int j = 0; j < g; ++j) { ip[j] = in + (size_t)(f0 + j) * in_stride; op[j] = out + (size_t)(f0 + j) * c->nc; }
+    MM_TRY(c->fft.forward<T>(ip, op, g, c->s, scale));
+  }
+  return 0;
+}
+template <typename T> int inv(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf) {
+  for (int f0 = 0; f0 < nf; f0 += kMaxBatch) {
+    const int g = std::min(kMaxBatch, nf - f0);
+    MM_TRY(c->fft.inverse<T>((void* const*)(sp + f0), out + f0, g, c->s));
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ eldis
+template <typename T> int eldis(mm_ctx* c) {
+  const size_t n3 = c->n3, nc = c->nc;
+  const Kern<T>& K = kern<T>(c);
+  const dim3 gp(c->nblk_pw), gk(c->nblk_k);
+  T* phi = (T*)c->phi;
+  T* strain = (T*)c->strain;
+  T* xpol = (T*)c->xpol;
+  C2<T>* work = (C2<T>*)c->swork;
+  C2<T>* uhat = (C2<T>*)c->suhat;
+  C2<T>* wsp[6];
+  T* wre[6];
+  for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = strain + (size_t)m * n3; }
+  if (c->step == c->prm.elastep0) {
+    K.ppoly(gp, c->s, phi, xpol, n3);
+    MM_HIP(hipGetLastError());
+    C2<T>* phat = work + (size_t)6 * nc;
+    MM_TRY(fwd<T>(c, xpol, n3, phat, c->V));
+    K.zeroth(gk, c->s, c->dp, phat, uhat, work, nc);
+    MM_HIP(hipGetLastError());
+    MM_TRY(inv<T>(c, wsp, wre, 6));
+  }
+  double newfac = 0.5, cfac = 1.0, err_old[6] = {0, 0, 0, 0, 0, 0};
+  int niter = 0, total = 0;
+  bool converged = false;
+  for (;;) {
+    niter += 1;
+    total += 1;
+    c->probe.begin(MM_PROBE_POLAR, c->s);
+    K.polar(gp, c->s, c->dp, phi, strain, cfac, xpol, c->derr, n3);
+    c->probe.end(MM_PROBE_POLAR, c->s, (double)n3 * sizeof(T) * (c->V + 12));
+    MM_HIP(hipGetLastError());
+    MM_TRY(fwd<T>(c, xpol, n3, work, 6));
+    c->probe.begin(MM_PROBE_KS_ELASTIC, c->s);
+    hipLaunchKernelGGL(k_ks_elastic<T>, gk, dim3(kBlock), 0, c->s, c->dp, work, uhat, c->part, nc);
+    c->probe.end(MM_PROBE_KS_ELASTIC, c->s, (double)nc * 2 * sizeof(T) * 18);
+    MM_HIP(hipGetLastError());
+    MM_TRY(finalize(c, c->nblk_k, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
+    MM_HIP(hipMemcpyAsync(c->hres + R_ERR0, c->dres + R_ERR0, 6 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    MM_HIP(hipEventRecord(c->ev, c->s));
+    MM_TRY(inv<T>(c, wsp, wre, 6));
+    MM_HIP(hipEventSynchronize(c->ev));
+    const double* err = c->hres + R_ERR0;
+    // convergence / backtracking control, ModElastic.f:435-481
+    if (err[0] / err[1] < 1.e-7 && err[2] / err[3] < 1.e-7 && err[4] / err[5] < 1.e-7) converged = true;
+    if (niter > 1 && !converged) {
+      if (err[0] / err[1] > err_old[0] / err_old[1] || err[2] / err[3] > err_old[2] / err_old[3] ||
+          err[4] / err[5] > err_old[4] / err_old[5]) {
+        cfac = cfac * newfac;
+        newfac = newfac * 0.5;
+        if (newfac < 1.e-7) return MM_ERR_NEWFAC;
+        niter = 0;
+        converged = false;
+        continue;
+      }
+    }
+    for (int m = 0; m < 6; ++m) err_old[m] = err[m];
+    if (niter == 10) converged = true;
+    if (!converged || niter < 4) continue;
+    if (cfac < 1.0 - 1.e-6) {
+      cfac = cfac + 0.1;
+      if (cfac > 1.0) cfac = 1.0;
+      niter = 0;
+      converged = false;
+      continue;
+    }
+    break;
+  }
+  c->last_niter = total;
+  c->last_cfac = cfac;
+  c->probe.begin(MM_PROBE_ELENER, c->s);
+  K.elener(gp, c->s, c->dp, phi, strain, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->derr, n3);
+  c->probe.end(MM_PROBE_ELENER, c->s, (double)n3 * sizeof(T) * (4 * c->V + 7));
+  MM_HIP(hipGetLastError());
+  MM_TRY(finalize(c, c->nblk_pw, 1, {0}, R_EL));
+  return 0;
+}
+
+// -------------------------------------------------------------- one step
+template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
+  const size_t n3 = c->n3, nc = c->nc;
+  const int V = c->V, E = c->E;
+  const Kern<T>& K = kern<T>(c);
+  const dim3 gp(c->nblk_pw);
+  T* phi = (T*)c->phi;
+  C2<T>* sphi = (C2<T>*)c->sphi;
+  C2<T>* work = (C2<T>*)c->swork;
+  if (c->step <= c->prm.noise_step) return MM_ERR_ARG;  // Langevin noise is host-side, not supported here
+  MM_HIP(hipMemsetAsync(c->derr, 0, sizeof(int), c->s));
+  // 1. phi_hat (Multi_MEUMAPPS.f:577-578); kept for the Allen-Cahn update
+  MM_TRY(fwd<T>(c, phi, n3, sphi, V));
+  // 2. gradient block (:581-613) in groups of up to 4 variants
+  const int G = std::max(1, std::min({V, c->nwork / 4, 4}));
+  for (int v0 = 0; v0 < V; v0 += G) {
+    const int g = std::min(G, V - v0);
+    hipLaunchKernelGGL(k_ks_grad<T>, dim3(c->nblk_k, g), dim3(kBlock), 0, c->s, c->dp, sphi, work, v0, nc);
+    MM_HIP(hipGetLastError());
+    C2<T>* sp[16];
+    T* rp[16];
+    for (int j = 0; j < g; ++j) {
+      const int v = v0 + j;
+      for (int dd = 0; dd < 3; ++dd) { sp[4 * j + dd] = work + (size_t)(4 * j + dd) * nc; rp[4 * j + dd] = (T*)c->gphi + (size_t)(3 * v + dd) * n3; }
+      sp[4 * j + 3] = work + (size_t)(4 * j + 3) * nc;
+      rp[4 * j + 3] = (T*)c->kdel + (size_t)v * n3;
+    }
+    MM_TRY(inv<T>(c, sp, rp, 4 * g));
+  }
+  // 3. chemistry (:634-845)
+  c->probe.begin(MM_PROBE_CHEM, c->s);
+  K.chem(gp, c->s, c->dp, phi, (T*)c->gphi, (T*)c->kdel, (T*)c->cons, (T*)c->fdiff, (T*)c->flux, c->part,
+         c->derr, n3);
+  c->probe.end(MM_PROBE_CHEM, c->s, (double)n3 * sizeof(T) * (6 * V + 4 * E));
+  MM_HIP(hipGetLastError());
+  MM_TRY(finalize(c, c->nblk_pw, 3, {0, 0, 0}, R_FINT));
+  // 4. elasticity (:851-880)
+  c->last_niter = 0;
+  c->last_cfac = 1.0;
+  if (c->step >= c->prm.elastep0) {
+    MM_TRY(eldis<T>(c));
+  } else {
+    MM_HIP(hipMemsetAsync(c->el_en, 0, n3 * sizeof(T), c->s));
+    MM_HIP(hipMemsetAsync(c->elgrad, 0, (size_t)V * n3 * sizeof(T), c->s));
+    MM_HIP(hipMemsetAsync(c->dres + R_EL, 0, sizeof(double), c->s));
+  }
+  // 5. Cahn-Hilliard (:910-949): work[4e] = c_hat, work[4e+1..3] = F_hat
+  {
+    const T scale = (T)(1.0 / (double)(float)n3);
+    for (int e = 0; e < E; ++e) {
+      const T* ip[4] = {(T*)c->cons + (size_t)e * n3, (T*)c->flux + (size_t)(3 * e) * n3,
+                        (T*)c->flux + (size_t)(3 * e + 1) * n3, (T*)c->flux + (size_t)(3 * e + 2) * n3};
+      void* op[4] = {work + (size_t)(4 * e) * nc, work + (size_t)(4 * e + 1) * nc, work + (size_t)(4 * e + 2) * nc,
+                     work + (size_t)(4 * e + 3) * nc};
+      MM_TRY(c->fft.forward<T>(ip, op, 4, c->s, scale));
+    }
+    hipLaunchKernelGGL(k_ks_ch<T>, dim3(c->nblk_k, E), dim3(kBlock), 0, c->s, c->dp, work, nc);
+    MM_HIP(hipGetLastError());
+    C2<T>* sp[MM_MAX_EL];
+    T* rp[MM_MAX_EL];
+    for (int e = 0; e < E; ++e) { sp[e] = work + (size_t)(4 * e) * nc; rp[e] = (T*)c->cons + (size_t)e * n3; }
+    MM_TRY(inv<T>(c, sp, rp, E));
+  }
+  // 6. Allen-Cahn (:955-1001)
+  {
+    MM_TRY(fwd<T>(c, (T*)c->fdiff, n3, work, V));
+    hipLaunchKernelGGL(k_ks_ac<T>, dim3(c->nblk_k, V), dim3(kBlock), 0, c->s, c->dp, sphi, work, nc);
+    MM_HIP(hipGetLastError());
+    C2<T>* sp[MM_MAX_VAR];
+    T* rp[MM_MAX_VAR];
+    for (int v = 0; v < V; ++v) { sp[v] = sphi + (size_t)v * nc; rp[v] = phi + (size_t)v * n3; }
+    MM_TRY(inv<T>(c, sp, rp, V));
+  }
+  // 7. clamp and check-loop sums (:1008-1065)
+  K.clamp(gp, c->s, c->dp, phi, (T*)c->cons, c->part, n3);
+  MM_HIP(hipGetLastError());
+  {
+    FinalizeOps fo;
+    for (int e = 0; e < E; ++e) fo.op[e] = 0;
+    fo.op[E] = 1; fo.op[E + 1] = 2; fo.op[E + 2] = 2; fo.op[E + 3] = 0;
+    for (int i = E + 4; i < 8; ++i) fo.op[i] = 0;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->s, c->part, c->nblk_pw, E + 4, fo, c->dres + R_CLAMP);
+    MM_HIP(hipGetLastError());
+  }
+  MM_HIP(hipMemcpyAsync(c->hres, c->dres, R_N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  int herr = 0;
+  MM_HIP(hipMemcpyAsync(c->hres + R_N, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  MM_HIP(hipStreamSynchronize(c->s));
+  MM_TRY(c->probe.collect());
+  std::memcpy(&herr, c->hres + R_N, sizeof(int));
+  if (herr) return herr;
+  const double* r = c->hres;
+  if (d) {
+    std::memset(d, 0, sizeof(*d));
+    d->step = c->step;
+    d->n_elastic = c->last_niter;
+    d->cfac = c->last_cfac;
+    d->fint = r[R_FINT];
+    d->fch = r[R_FCH];
+    d->fw = r[R_FW];
+    d->elen = r[R_EL];
+    d->etot = d->fint + d->fch + d->elen + d->fw;
+    d->glob_min = c->glob_min;
+    d->glob_max = c->glob_max;
+    d->phi_max = c->phi_max;
+    d->solid_tot = c->solid_frac / (double)(float)n3;
+    for (int e = 0; e < E; ++e)
+      d->caviter[e] = c->hres[R_CLAMP + e] * (1.0 / (double)(float)n3) + c->hp.cbar[e];
+  }
+  // lagged check-loop values for the next row (con_min starts at 1, con_max and max_phi at 0)
+  c->glob_min = std::min(1.0, r[R_CLAMP + E]);
+  c->glob_max = std::max(0.0, r[R_CLAMP + E + 1]);
+  c->phi_max = std::max(0.0, r[R_CLAMP + E + 2]);
+  c->solid_frac = r[R_CLAMP + E + 3];
+  c->step += 1;
+  return 0;
+}
+
+void free_all(mm_ctx* c) {
+  void* bufs[] = {c->phi, c->cons, c->gphi, c->kdel, c->fdiff, c->flux, c->strain, c->xpol, c->el_en,
+                  c->elgrad, c->sphi, c->suhat, c->swork, c->part, c->dres, c->derr, c->dp};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->hres) (void)hipHostFree(c->hres);
+  if (c->ev) (void)hipEventDestroy(c->ev);
+  c->probe.destroy();
+  if (c->s) (void)hipStreamDestroy(c->s);
+  c->fft.destroy();
+}
+
+// upload helpers ---------------------------------------------------------
+template <typename T> int put_field(mm_ctx* c, void* dst, const std::vector<double>& h) {
+  if (sizeof(T) == 8) {
+    MM_HIP(hipMemcpy(dst, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(h.size());
+    for (size_t i = 0; i < h.size(); ++i) f[i] = (float)h[i];
+    MM_HIP(hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+template <typename T> int get_field(mm_ctx* c, const void* src, std::vector<double>& h) {
+  if (sizeof(T) == 8) {
+    MM_HIP(hipMemcpy(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> f(h.size());
+    MM_HIP(hipMemcpy(f.data(), src, f.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); ++i) h[i] = (double)f[i];
+  }
+  return 0;
+}
+
+template <typename T>
+int upload(mm_ctx* c, const double* phi, const double* cons, const double* el_en, const double* u,
+           const double* gradu) {
+  const size_t n3 = c->n3;
+  const int V = c->V, E = c->E;
+  if (phi) {
+    std::vector<double> h((size_t)V * n3);
+    for (size_t i = 0; i < n3; ++i)
+      for (int v = 0; v < V; ++v) h[(size_t)v * n3 + i] = phi[i * V + v];
+    MM_TRY(put_field<T>(c, c->phi, h));
+  }
+  if (cons) {
+    std::vector<double> h((size_t)E * n3);
+    for (int e = 0; e < E; ++e) {
+      double mean = 0.0;
+      if (c->prec == MM_MIXED) {
+        for (size_t i = 0; i < n3; ++i) mean += cons[i * E + e];
+        mean /= (double)n3;
+      }
+      c->hp.cbar[e] = mean;
+      for (size_t i = 0; i < n3; ++i) h[(size_t)e * n3 + i] = cons[i * E + e] - mean;
+    }
+    MM_TRY(put_field<T>(c, c->cons, h));
+    MM_HIP(hipMemcpy(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice));
+  }
+  if (el_en) {
+    std::vector<double> h(el_en, el_en + n3);
+    MM_TRY(put_field<T>(c, c->el_en, h));
+  }
+  if (gradu) {
+    // engineering strain from the nine gradients (record order d_1 u_1, d_1 u_2, d_1 u_3, d_2 u_1, ...)
+    std::vector<double> h(6 * n3);
+    auto g = [&](int dd, int i, size_t p) { return gradu[(size_t)(dd * 3 + i) * n3 + p]; };
+    for (size_t p = 0; p < n3; ++p) {
+      h[0 * n3 + p] = g(0, 0, p);
+      h[1 * n3 + p] = g(1, 1, p);
+      h[2 * n3 + p] = g(2, 2, p);
+      h[3 * n3 + p] = g(1, 2, p) + g(2, 1, p);
+      h[4 * n3 + p] = g(0, 2, p) + g(2, 0, p);
+      h[5 * n3 + p] = g(1, 0, p) + g(0, 1, p);
+    }
+    MM_TRY(put_field<T>(c, c->strain, h));
+  }
+  if (u) {
+    std::vector<double> h(u, u + 3 * n3);
+    MM_TRY(put_field<T>(c, c->xpol, h));
+    MM_TRY(fwd<T>(c, (T*)c->xpol, n3, (C2<T>*)c->suhat, 3));
+    MM_HIP(hipStreamSynchronize(c->s));
+  }
+  return 0;
+}
+
+template <typename T>
+int download(mm_ctx* c, double* phi, double* cons, double* el_en, double* u, double* gradu) {
+  const size_t n3 = c->n3, nc = c->nc;
+  const int V = c->V, E = c->E;
+  MM_HIP(hipStreamSynchronize(c->s));
+  if (phi) {
+    std::vector<double> h((size_t)V * n3);
+    MM_TRY(get_field<T>(c, c->phi, h));
+    for (size_t i = 0; i < n3; ++i)
+      for (int v = 0; v < V; ++v) phi[i * V + v] = h[(size_t)v * n3 + i];
+  }
+  if (cons) {
+    std::vector<double> h((size_t)E * n3);
+    MM_TRY(get_field<T>(c, c->cons, h));
+    for (size_t i = 0; i < n3; ++i)
+      for (int e = 0; e < E; ++e) cons[i * E + e] = c->hp.cbar[e] + h[(size_t)e * n3 + i];
+  }
+  if (el_en) {
+    std::vector<double> h(n3);
+    MM_TRY(get_field<T>(c, c->el_en, h));
+    std::memcpy(el_en, h.data(), n3 * 8);
+  }
+  C2<T>* work = (C2<T>*)c->swork;
+  C2<T>* uh = (C2<T>*)c->suhat;
+  T* scratch = (T*)c->xpol;
+  if (u) {
+    MM_HIP(hipMemcpyAsync(work, uh, 3 * nc * sizeof(C2<T>), hipMemcpyDeviceToDevice, c->s));
+    C2<T>* sp[3] = {work, work + nc, work + 2 * nc};
+    T* rp[3] = {scratch, scratch + n3, scratch + 2 * n3};
+    MM_TRY(inv<T>(c, sp, rp, 3));
+    MM_HIP(hipStreamSynchronize(c->s));
+    std::vector<double> h(3 * n3);
+    MM_TRY(get_field<T>(c, scratch, h));
+    std::memcpy(u, h.data(), 3 * n3 * 8);
+  }
+  if (gradu) {
+    for (int dd = 0; dd < 3; ++dd) {
+      for (int i = 0; i < 3; ++i)
+        hipLaunchKernelGGL(k_ks_deriv<T>, dim3(c->nblk_k), dim3(kBlock), 0, c->s, c->dp, uh + (size_t)i * nc,
+                           work + (size_t)i * nc, dd, nc);
+      MM_HIP(hipGetLastError());
+      C2<T>* sp[3] = {work, work + nc, work + 2 * nc};
+      T* rp[3] = {scratch, scratch + n3, scratch + 2 * n3};
+      MM_TRY(inv<T>(c, sp, rp, 3));
+      MM_HIP(hipStreamSynchronize(c->s));
+      std::vector<double> h(3 * n3);
+      MM_TRY(get_field<T>(c, scratch, h));
+      std::memcpy(gradu + (size_t)dd * 3 * n3, h.data(), 3 * n3 * 8);
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
+  if (!p || !out) return MM_ERR_ARG;
+  *out = nullptr;
+  if (precision != MM_FP64 && precision != MM_MIXED) return MM_ERR_ARG;
+  if (p->vartot < 1 || p->vartot > MM_MAX_VAR || p->numel < 1 || p->numel > MM_MAX_EL || p->numph < 2 ||
+      p->numph > MM_MAX_PH)
+    return MM_ERR_ARG;
+  int s = 0;
+  for (int q = 0; q < p->numph; ++q) s += p->nvar[q];
+  if (s != p->vartot + 1 || p->nvar[0] != 1) return MM_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MM_ERR_NODEV;
+  mm_ctx* c = new mm_ctx();
+  c->prm = *p;
+  c->prec = precision;
+  c->V = p->vartot; c->E = p->numel; c->NPH = p->numph;
+  c->step = p->step;
+  bool ok = precision == MM_FP64 ? kern_table<double>(c->V, c->NPH, c->E, c->k64)
+                                 : kern_table<float>(c->V, c->NPH, c->E, c->k32);
+  int rc = ok ? c->fft.init(p->nx, p->ny, p->nz, precision) : MM_ERR_ARG;
+  c->fft.probe = &c->probe;
+  if (rc) { free_all(c); delete c; return rc; }
+  c->n3 = c->fft.nreal();
+  c->nc = c->fft.ncplx();
+  c->esz = precision == MM_FP64 ? 8 : 4;
+  host_params(*p, precision, c->hp);
+  const size_t n3 = c->n3, nc = c->nc, rb = c->esz, cb = 2 * c->esz;
+  const int V = c->V, E = c->E;
+  c->nwork = std::max({6 + V, 4 * std::min(V, 4), 4 * E, 6});
+  c->nxpol = std::max(6, V);
+  c->nblk_pw = blocks_for(n3);
+  c->nblk_k = blocks_for(nc);
+  rc = 0;
+  rc = rc ? rc : dalloc(&c->phi, V * n3 * rb);
+  rc = rc ? rc : dalloc(&c->cons, E * n3 * rb);
+  rc = rc ? rc : dalloc(&c->gphi, 3 * V * n3 * rb);
+  rc = rc ? rc : dalloc(&c->kdel, V * n3 * rb);
+  rc = rc ? rc : dalloc(&c->fdiff, V * n3 * rb);
+  rc = rc ? rc : dalloc(&c->flux, 3 * E * n3 * rb);
+  rc = rc ? rc : dalloc(&c->strain, 6 * n3 * rb);
+  rc = rc ? rc : dalloc(&c->xpol, c->nxpol * n3 * rb);
+  rc = rc ? rc : dalloc(&c->el_en, n3 * rb);
+  rc = rc ? rc : dalloc(&c->elgrad, V * n3 * rb);
+  rc = rc ? rc : dalloc(&c->sphi, V * nc * cb);
+  rc = rc ? rc : dalloc(&c->suhat, 3 * nc * cb);
+  rc = rc ? rc : dalloc(&c->swork, c->nwork * nc * cb);
+  rc = rc ? rc : dalloc((void**)&c->part, (size_t)kMaxPartSlots * kMaxBlocks * sizeof(double));
+  rc = rc ? rc : dalloc((void**)&c->dres, 64 * sizeof(double));
+  rc = rc ? rc : dalloc((void**)&c->derr, 16);
+  rc = rc ? rc : dalloc((void**)&c->dp, sizeof(DevParams));
+  if (!rc && hipHostMalloc((void**)&c->hres, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = MM_ERR_NOMEM;
+  if (!rc && hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) rc = MM_ERR_HIP;
+  if (!rc && hipEventCreateWithFlags(&c->ev, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
+  if (!rc && hipMemcpy(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess) rc = MM_ERR_HIP;
+  if (!rc) {
+    // zero state so that a step without upload is well defined
+    void* z[] = {c->phi, c->cons, c->strain, c->el_en, c->elgrad, c->suhat};
+    size_t zb[] = {V * n3 * rb, E * n3 * rb, 6 * n3 * rb, n3 * rb, V * n3 * rb, 3 * nc * cb};
+    for (int i = 0; i < 6 && !rc; ++i)
+      if (hipMemset(z[i], 0, zb[i]) != hipSuccess) rc = MM_ERR_HIP;
+  }
+  if (rc) { free_all(c); delete c; return rc; }
+  *out = c;
+  return 0;
+}
+
+extern "C" void mm_destroy(mm_ctx* c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->s);
+  free_all(c);
+  delete c;
+}
+
+extern "C" int mm_upload(mm_ctx* c, const double* phi, const double* cons, const double* el_en, const double* u,
+                         const double* gradu) {
+  if (!c) return MM_ERR_ARG;
+  return c->prec == MM_FP64 ? upload<double>(c, phi, cons, el_en, u, gradu) : upload<float>(c, phi, cons, el_en, u, gradu);
+}
+
+extern "C" int mm_download(mm_ctx* c, double* phi, double* cons, double* el_en, double* u, double* gradu) {
+  if (!c) return MM_ERR_ARG;
+  return c->prec == MM_FP64 ? download<double>(c, phi, cons, el_en, u, gradu)
+                            : download<float>(c, phi, cons, el_en, u, gradu);
+}
+
+extern "C" int mm_step(mm_ctx* c, int32_t nsteps, mm_diag* diag) {
+  if (!c || nsteps < 0) return MM_ERR_ARG;
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = c->prec == MM_FP64 ? run_step<double>(c, diag ? diag + i : nullptr)
+                                : run_step<float>(c, diag ? diag + i : nullptr);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int mm_get_step(const mm_ctx* c) { return c ? c->step : 0; }
+extern "C" int mm_set_step(mm_ctx* c, int32_t step) {
+  if (!c) return MM_ERR_ARG;
+  c->step = step;
+  return 0;
+}
+
+extern "C" int mm_field(mm_ctx* c, int32_t which, void** dev, int32_t* esz, int32_t* nf) {
+  if (!c || !dev) return MM_ERR_ARG;
+  void* p = nullptr;
+  int n = 0;
+  switch (which) {
+    case 0: p = c->phi; n = c->V; break;
+    case 1: p = c->cons; n = c->E; break;
+    case 2: p = c->el_en; n = 1; break;
+    case 3: p = c->fdiff; n = c->V; break;
+    case 4: p = c->strain; n = 6; break;
+    case 5: p = c->gphi; n = 3 * c->V; break;
+    case 6: p = c->kdel; n = c->V; break;
+    case 7: p = c->elgrad; n = c->V; break;
+    default: return MM_ERR_ARG;
+  }
+  *dev = p;
+  if (esz) *esz = (int32_t)c->esz;
+  if (nf) *nf = n;
+  return 0;
+}
+
+extern "C" int mm_field_to_host(mm_ctx* c, int32_t which, double* host) {
+  void* p;
+  int32_t esz, n;
+  MM_TRY(mm_field(c, which, &p, &esz, &n));
+  if (!host) return MM_ERR_ARG;
+  MM_HIP(hipStreamSynchronize(c->s));
+  std::vector<double> h((size_t)n * c->n3);
+  if (c->prec == MM_FP64) { MM_TRY(get_field<double>(c, p, h)); } else { MM_TRY(get_field<float>(c, p, h)); }
+  if (which == 1)
+    for (int e = 0; e < c->E; ++e)
+      for (size_t i = 0; i < c->n3; ++i) h[(size_t)e * c->n3 + i] += c->hp.cbar[e];
+  std::memcpy(host, h.data(), h.size() * 8);
+  return 0;
+}
+
+extern "C" int mm_probe(mm_ctx* c, int32_t kind) {
+  if (!c || kind < -1 || kind > MM_PROBE_KS_ELASTIC) return MM_ERR_ARG;
+  MM_HIP(hipStreamSynchronize(c->s));
+  MM_TRY(c->probe.collect());
+  c->probe.reset();
+  c->probe.kind = kind;
+  return 0;
+}
+
+extern "C" int mm_probe_read(mm_ctx* c, double* ms, int64_t* launches, double* bytes) {
+  if (!c) return MM_ERR_ARG;
+  MM_HIP(hipStreamSynchronize(c->s));
+  MM_TRY(c->probe.collect());
+  if (ms) *ms = c->probe.ms;
+  if (launches) *launches = c->probe.launches;
+  if (bytes) *bytes = c->probe.bytes;
+  const int kind = c->probe.kind;
+  c->probe.reset();
+  c->probe.kind = kind;
+  return 0;
+}

@@ -1,0 +1,120 @@
+"""Host-side mirror of the reference driver around the C ABI.
+
+``Simulation`` plays the part of the Fortran main program's loop
+(Multi_MEUMAPPS.f:548-1106): it owns an ``mm_ctx`` (the whole grid resident
+in HBM), uploads the restart record set, advances steps and returns the
+fort.98 / fort.99 rows.  Arrays are numpy, structure-of-arrays with shape
+(ncomp, Nz, Ny, Nx) (x fastest); conversion to the reference's component-
+fastest Fortran layout happens here, at the boundary.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib
+from .params import MM_FP64, MM_MIXED, MMDiag, MMParams
+
+FIELD = {"phi": 0, "cons": 1, "el_en": 2, "f_diff": 3, "strain": 4, "grad_phi": 5,
+         "k_del_sq": 6, "el_grad": 7}
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def soa_to_aos(a: np.ndarray) -> np.ndarray:
+    """(ncomp, nz, ny, nx) -> Fortran a(ncomp, Nx, Ny, Nz) memory order."""
+    return np.ascontiguousarray(np.moveaxis(a, 0, -1), dtype=np.float64)
+
+
+def aos_to_soa(a: np.ndarray, ncomp: int, shape) -> np.ndarray:
+    return np.ascontiguousarray(np.moveaxis(a.reshape(*shape, ncomp), -1, 0))
+
+
+class Simulation:
+    def __init__(self, params: MMParams, precision: int = MM_FP64):
+        self.lib = _lib.load()
+        self.p = params
+        self.precision = precision
+        self.ctx = ctypes.c_void_p()
+        _lib.check(self.lib.mm_create(ctypes.byref(params), precision, ctypes.byref(self.ctx)), "mm_create")
+        self.shape = (params.nz, params.ny, params.nx)
+        self.V, self.E = params.vartot, params.numel
+
+    def close(self):
+        if self.ctx:
+            self.lib.mm_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- state
+    def upload(self, phi=None, cons=None, u=None, gradu=None, el_en=None):
+        phi_a = None if phi is None else soa_to_aos(np.asarray(phi).reshape(self.V, *self.shape))
+        cons_a = None if cons is None else soa_to_aos(np.asarray(cons).reshape(self.E, *self.shape))
+        u_a = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        g_a = None if gradu is None else np.ascontiguousarray(gradu, dtype=np.float64)
+        e_a = None if el_en is None else np.ascontiguousarray(el_en, dtype=np.float64)
+        _lib.check(self.lib.mm_upload(self.ctx, _ptr(phi_a), _ptr(cons_a), _ptr(e_a), _ptr(u_a), _ptr(g_a)),
+                   "mm_upload")
+
+    def download(self):
+        n3 = int(np.prod(self.shape))
+        phi = np.empty(self.V * n3); cons = np.empty(self.E * n3)
+        el = np.empty(self.shape); u = np.empty((3, *self.shape)); gu = np.empty((9, *self.shape))
+        _lib.check(self.lib.mm_download(self.ctx, _ptr(phi), _ptr(cons), _ptr(el), _ptr(u), _ptr(gu)),
+                   "mm_download")
+        return dict(phi=aos_to_soa(phi, self.V, self.shape), cons=aos_to_soa(cons, self.E, self.shape),
+                    el_en=el, u=u, gradu=gu)
+
+    def field(self, name: str) -> np.ndarray:
+        which = FIELD[name]
+        ptr = ctypes.c_void_p(); esz = ctypes.c_int32(); nf = ctypes.c_int32()
+        _lib.check(self.lib.mm_field(self.ctx, which, ctypes.byref(ptr), ctypes.byref(esz), ctypes.byref(nf)),
+                   "mm_field")
+        out = np.empty((nf.value, *self.shape))
+        _lib.check(self.lib.mm_field_to_host(self.ctx, which, _ptr(out)), "mm_field_to_host")
+        return out
+
+    def device_field(self, name: str):
+        """(device address, element size, number of fields) of a resident field."""
+        ptr = ctypes.c_void_p(); esz = ctypes.c_int32(); nf = ctypes.c_int32()
+        _lib.check(self.lib.mm_field(self.ctx, FIELD[name], ctypes.byref(ptr), ctypes.byref(esz),
+                                     ctypes.byref(nf)), "mm_field")
+        return ptr.value, esz.value, nf.value
+
+    @property
+    def step_count(self) -> int:
+        return self.lib.mm_get_step(self.ctx)
+
+    def set_step(self, s: int):
+        _lib.check(self.lib.mm_set_step(self.ctx, s), "mm_set_step")
+
+    # ----------------------------------------------------------------- step
+    def step(self, n: int = 1, diag: bool = True) -> List[dict]:
+        if not diag:
+            _lib.check(self.lib.mm_step(self.ctx, n, None), "mm_step")
+            return []
+        rows = (MMDiag * n)()
+        _lib.check(self.lib.mm_step(self.ctx, n, rows), "mm_step")
+        return [r.as_dict(self.E) for r in rows]
+
+
+def fort98_line(d: dict) -> str:
+    """One fort.98 row, format (i8,10e15.7) (Multi_MEUMAPPS.f:905-906,1113)."""
+    vals = [d["fint"], d["fw"], d["fch"], d["elen"], d["etot"], d["glob_min"], d["glob_max"],
+            d["phi_max"], d["solid_tot"]]
+    return f"{d['step']:8d}" + "".join(f"{v:15.7E}" for v in vals)

@@ -1,0 +1,158 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of ``oracle/liboracle.so``.
+
+Fields are numpy arrays in structure-of-arrays form, shape (ncomp, nz, ny, nx)
+(x fastest), matching the oracle's C layout.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_void_p
+
+import numpy as np
+
+from meumapps_ss_amd.params import MMDiag, MMParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> str:
+    r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stderr)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = ctypes.CDLL(LIB)
+    D = POINTER(c_double)
+    sig = {
+        "oracle_create": (c_void_p, [POINTER(MMParams), c_int]),
+        "oracle_destroy": (None, [c_void_p]),
+        "oracle_set_threads": (c_int, [c_void_p, c_int]),
+        "oracle_set_state": (c_int, [c_void_p, D, D, D, D, D]),
+        "oracle_get_state": (c_int, [c_void_p, D, D, D, D, D]),
+        "oracle_get_work": (c_int, [c_void_p, c_int, D]),
+        "oracle_get_step": (c_int, [c_void_p]),
+        "oracle_set_step": (None, [c_void_p, c_int]),
+        "oracle_step": (c_int, [c_void_p, POINTER(MMDiag)]),
+        "oracle_f_trans": (c_int, [c_void_p, D, D]),
+        "oracle_inv_trans": (c_int, [c_void_p, D, D]),
+        "oracle_invert": (c_int, [D, c_int, c_int]),
+        "oracle_kaxis": (None, [c_int, c_int, D, D]),
+        "oracle_eldis": (c_int, [c_void_p, POINTER(c_int), D]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(POINTER(c_double))
+
+
+class Oracle:
+    def __init__(self, params: MMParams, nthreads: int = 1):
+        self.lib = load()
+        self.p = params
+        self.h = self.lib.oracle_create(ctypes.byref(params), nthreads)
+        if not self.h:
+            raise ValueError("oracle_create rejected the parameters")
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self.V, self.E = params.vartot, params.numel
+        self.shape = (self.nz, self.ny, self.nx)
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_threads(self, n):
+        self.lib.oracle_set_threads(self.h, n)
+
+    def set_state(self, phi=None, cons=None, u=None, gradu=None, el_en=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        phi, cons, u, gradu, el_en = map(conv, (phi, cons, u, gradu, el_en))
+        self.lib.oracle_set_state(self.h, _p(phi), _p(cons), _p(u), _p(gradu), _p(el_en))
+
+    def get_state(self):
+        V, E = self.V, self.E
+        phi = np.empty((V, *self.shape)); cons = np.empty((E, *self.shape))
+        u = np.empty((3, *self.shape)); gu = np.empty((9, *self.shape)); el = np.empty(self.shape)
+        self.lib.oracle_get_state(self.h, _p(phi), _p(cons), _p(u), _p(gu), _p(el))
+        return dict(phi=phi, cons=cons, u=u, gradu=gu, el_en=el)
+
+    def get_work(self, which):
+        n = {0: 3 * self.V, 1: self.V, 2: self.V, 3: self.V, 4: self.V, 5: 3 * self.E, 6: self.V}[which]
+        out = np.empty((n, *self.shape))
+        self.lib.oracle_get_work(self.h, which, _p(out))
+        return out
+
+    @property
+    def step_count(self):
+        return self.lib.oracle_get_step(self.h)
+
+    def set_step(self, s):
+        self.lib.oracle_set_step(self.h, s)
+
+    def step(self, n=1):
+        rows = []
+        for _ in range(n):
+            d = MMDiag()
+            rc = self.lib.oracle_step(self.h, ctypes.byref(d))
+            if rc:
+                raise RuntimeError(f"oracle_step error {rc}")
+            rows.append(d.as_dict(self.E))
+        return rows
+
+    def f_trans(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        H = self.nx // 2 + 1
+        out = np.empty((self.nz, self.ny, H), dtype=np.complex128)
+        self.lib.oracle_f_trans(self.h, _p(x), out.ctypes.data_as(POINTER(c_double)))
+        return out
+
+    def inv_trans(self, s):
+        s = np.ascontiguousarray(s, dtype=np.complex128)
+        out = np.empty(self.shape)
+        self.lib.oracle_inv_trans(self.h, s.ctypes.data_as(POINTER(c_double)), _p(out))
+        return out
+
+    def eldis(self):
+        n = c_int(0)
+        cf = c_double(0)
+        rc = self.lib.oracle_eldis(self.h, ctypes.byref(n), ctypes.byref(cf))
+        return rc, n.value, cf.value
+
+
+def invert(a: np.ndarray, flag: int = 1):
+    """oracle_invert on a copy of the (n,n) matrix; returns (rc, inverse)."""
+    lib = load()
+    n = a.shape[0]
+    f = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    rc = lib.oracle_invert(f.ctypes.data_as(POINTER(c_double)), n, flag)
+    return rc, np.array(f, order="C")
+
+
+def kaxis(n: int, count: int):
+    lib = load()
+    kf = np.empty(count); kq = np.empty(count)
+    lib.oracle_kaxis(n, count, _p(kf), _p(kq))
+    return kf, kq

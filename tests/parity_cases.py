@@ -1,0 +1,62 @@
+"""Shared parity cases: model inputs and seeded initial fields that exercise
+the branches of the step (clamp at 0 and 1, phi > 0 tests of the f_diff
+correction, overlapping variants, inhomogeneous moduli with a real elastic
+fixed-point iteration, a second precipitate phase, two compositions)."""
+from __future__ import annotations
+
+import numpy as np
+
+from meumapps_ss_amd.configs import model_inputs
+from meumapps_ss_amd.params import setup
+
+
+def smooth_field(rng, shape, lo, hi, cut=0.25):
+    """Random field with spectral content below cut*Nyquist, scaled to [lo, hi]."""
+    a = rng.standard_normal(shape)
+    f = np.fft.rfftn(a)
+    kz = np.fft.fftfreq(shape[0])[:, None, None]
+    ky = np.fft.fftfreq(shape[1])[None, :, None]
+    kx = np.fft.rfftfreq(shape[2])[None, None, :]
+    k = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    f[k > cut * 0.5] = 0.0
+    b = np.fft.irfftn(f, s=shape, axes=(0, 1, 2))
+    b = (b - b.min()) / (b.max() - b.min())
+    return lo + (hi - lo) * b
+
+
+CASES = {
+    # name: (n, nvar, tstr, dt, extra model kwargs, init kind, numel_in)
+    "v1_dilat": (16, 1, (0.01, 0.01, 0.01), 0.01, {}, "rich"),
+    "v3_tetra": (16, 3, (0.02, -0.01, -0.01), 0.005, {}, "rich"),
+    "v3_noise": (32, 3, (0.02, -0.01, -0.01), 0.005, {}, "noise"),
+    "v2_twophase": (16, 1, (0.02, 0.01, 0.01), 0.005, {"numph": 3}, "rich"),
+    "v3_inhomog": (16, 3, (0.02, 0.01, -0.01), 0.005,
+                   {"cela2": {(1, 1): 300.0, (1, 2): 140.0, (4, 4): 120.0}}, "rich"),
+    "v2_nonsq": (24, 2, (0.015, 0.015, -0.01), 0.005, {}, "rich"),
+    "v1_e2": (16, 1, (0.01, 0.01, 0.01), 0.005, {"numel_in": 3}, "rich"),
+}
+
+
+def make_case(name, seed=0):
+    n, nvar, tstr, dt, kw, init = CASES[name]
+    kw = dict(kw)
+    numel_in = kw.pop("numel_in", 2)
+    inp = model_inputs(n, nvar, tstr, dt, **kw)
+    if numel_in != 2:
+        # a second composition: A, G0, c0 rows extend to 2*numel+1
+        inp.numel_in = numel_in
+        for ph in inp.phases:
+            a, g0, c0 = ph.gparams
+            ph.gparams = [a, 0.5 * a, g0, c0, c0 - 0.1]
+        inp.conc = [(1, 0.4, 1.0, 0.0), (2, 0.25, 1.0, 0.0)]
+    p = setup(inp)
+    V, E = p.vartot, p.numel
+    shape = (n, n, n)
+    rng = np.random.default_rng(seed + 17 * n + 5 * V)
+    if init == "noise":
+        phi = rng.uniform(-0.01, 0.01, (V, *shape))
+        cons = 0.4 + rng.uniform(-0.01, 0.01, (E, *shape))
+    else:
+        phi = np.stack([smooth_field(rng, shape, -0.15, 1.1) for _ in range(V)])
+        cons = np.stack([smooth_field(rng, shape, 0.3, 0.6) for _ in range(E)])
+    return p, phi, cons

@@ -1,0 +1,131 @@
+"""FP64 GPU step against the CPU binary64 oracle on identical seeded inputs.
+
+Bar (north star): every order-parameter and composition field within 1e-10
+max-abs after the stated number of steps; the displacement records, elastic
+energy and intermediate fields of the first step to the same bar relative to
+their scale.  MIXED is checked against the FP64 GPU path: relative L2
+<= 1e-4 per field.
+"""
+import numpy as np
+import pytest
+
+from meumapps_ss_amd.params import MM_FP64, MM_MIXED
+from meumapps_ss_amd.sim import Simulation
+from oracle import Oracle
+from parity_cases import CASES, make_case
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_first_step_intermediates(gpu_lib, name):
+    p, phi, cons = make_case(name)
+    o = Oracle(p, 4)
+    o.set_state(phi=phi, cons=cons)
+    ref_rows = o.step(1)
+    with Simulation(p, MM_FP64) as s:
+        s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+        rows = s.step(1)
+        g = o.get_work(0)
+        V = p.vartot
+        gp = s.field("grad_phi")  # [v*3+d]
+        gp_o = np.stack([g[d * V + v] for v in range(V) for d in range(3)])
+        assert _rel(gp, gp_o) < TOL, "grad phi"
+        assert _rel(s.field("k_del_sq"), o.get_work(1)) < TOL, "k_del_sq"
+        assert _rel(s.field("el_grad"), o.get_work(4)) < TOL, "el_grad"
+        assert _rel(s.field("f_diff"), o.get_work(3)) < TOL, "f_diff"
+        st = s.download()
+        ro = o.get_state()
+        for k in ("phi", "cons"):
+            assert np.abs(st[k] - ro[k]).max() < TOL, k
+        for k in ("u", "gradu", "el_en"):
+            assert _rel(st[k], ro[k]) < TOL, k
+    r, q = rows[0], ref_rows[0]
+    assert r["n_elastic"] == q["n_elastic"]
+    for k in ("fint", "fch", "fw", "elen", "etot"):
+        assert abs(r[k] - q[k]) <= 1e-9 * max(1.0, abs(q[k])), k
+    assert abs(r["caviter"][0] - q["caviter"][0]) < 1e-13
+
+
+@pytest.mark.parametrize("name,nsteps", [("v3_tetra", 40), ("v3_inhomog", 20), ("v2_twophase", 30),
+                                         ("v3_noise", 60), ("v1_e2", 20)])
+def test_multi_step(gpu_lib, name, nsteps):
+    p, phi, cons = make_case(name, seed=1)
+    o = Oracle(p, 4)
+    o.set_state(phi=phi, cons=cons)
+    ref_rows = o.step(nsteps)
+    with Simulation(p, MM_FP64) as s:
+        s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+        rows = s.step(nsteps)
+        st = s.download()
+    ro = o.get_state()
+    for k in ("phi", "cons"):
+        err = np.abs(st[k] - ro[k]).max()
+        assert err < TOL, f"{k} max-abs {err:.3e}"
+    assert [r["n_elastic"] for r in rows] == [q["n_elastic"] for q in ref_rows]
+    # Cahn-Hilliard conserves mass to round-off
+    c0 = cons.mean(axis=(1, 2, 3))
+    for r in rows:
+        for e in range(p.numel):
+            assert abs(r["caviter"][e] - c0[e]) < 1e-13
+
+
+def test_elastep0_later(gpu_lib):
+    """Elasticity switched on after two steps (step < elastep0 path, then zeroth order)."""
+    p, phi, cons = make_case("v3_tetra", seed=2)
+    p.elastep0 = p.step + 2
+    o = Oracle(p, 4)
+    o.set_state(phi=phi, cons=cons)
+    o.step(5)
+    with Simulation(p, MM_FP64) as s:
+        s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+        s.step(5)
+        st = s.download()
+    ro = o.get_state()
+    for k in ("phi", "cons"):
+        assert np.abs(st[k] - ro[k]).max() < TOL
+    assert _rel(st["u"], ro["u"]) < TOL
+
+
+def test_restart_warm_start(gpu_lib):
+    """Upload u and its gradients (restart records) with step > elastep0."""
+    p, phi, cons = make_case("v3_tetra", seed=3)
+    o = Oracle(p, 4)
+    o.set_state(phi=phi, cons=cons)
+    o.step(3)
+    mid = o.get_state()
+    o.step(3)
+    ro = o.get_state()
+    p2 = p
+    p2.step = p.step + 3
+    with Simulation(p2, MM_FP64) as s:
+        s.upload(phi=mid["phi"], cons=mid["cons"], u=mid["u"], gradu=mid["gradu"], el_en=mid["el_en"])
+        s.step(3)
+        st = s.download()
+    for k in ("phi", "cons"):
+        assert np.abs(st[k] - ro[k]).max() < TOL
+    assert _rel(st["gradu"], ro["gradu"]) < TOL
+
+
+def test_mixed_vs_fp64(gpu_lib):
+    p, phi, cons = make_case("v3_noise", seed=4)
+    out = {}
+    for prec in (MM_FP64, MM_MIXED):
+        with Simulation(p, prec) as s:
+            s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+            rows = s.step(40)
+            out[prec] = (s.download(), rows)
+    a, b = out[MM_FP64][0], out[MM_MIXED][0]
+    for k in ("phi", "cons"):
+        l2 = np.linalg.norm(b[k] - a[k]) / np.linalg.norm(a[k])
+        assert l2 < 1e-4, f"{k} rel L2 {l2:.2e}"
+    # FP64 k=0 handling keeps the mean composition to round-off
+    c0 = cons.mean()
+    for r in out[MM_MIXED][1]:
+        assert abs(r["caviter"][0] - c0) < 1e-10
