@@ -91,8 +91,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--probe", default="z", choices=["x_r2c", "y", "z", "x_c2r", "chem", "polar",
-                                                      "elener", "ks_elastic"])
+    ap.add_argument("--probe", default="zf_elastic",
+                    choices=["x_r2c", "y", "z", "x_c2r", "chem", "polar", "elener", "ks_elastic",
+                             "zf_elastic", "zf_other"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build keeps the whole grid on a single MI355X "
@@ -110,7 +111,8 @@ def main():
     p = w.params()
     phi, cons = initial_fields(w)
     n3 = args.size ** 3
-    kinds = {"x_r2c": 0, "y": 1, "z": 2, "x_c2r": 3, "chem": 4, "polar": 5, "elener": 6, "ks_elastic": 7}
+    kinds = {"x_r2c": 0, "y": 1, "z": 2, "x_c2r": 3, "chem": 4, "polar": 5, "elener": 6, "ks_elastic": 7,
+             "zf_elastic": 8, "zf_other": 9}
     kind = kinds[args.probe]
 
     r64 = run_gpu(p, phi, cons, MM_FP64, args.steps, args.warmup, kind)

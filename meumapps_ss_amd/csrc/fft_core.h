@@ -137,15 +137,19 @@ template <typename T, int DIR> struct Dft<T, DIR, 16> : DftSplit<T, DIR, 16> {};
 
 // ------------------------------------------------------------ line plans
 // L = R0*R1*R2*R3, E points per thread (a multiple of every radix).
-template <int L> struct Plan;
-#define MM_PLAN(L_, E_, R0_, R1_, R2_, R3_)                                   \
-  template <> struct Plan<L_> {                                               \
-    static constexpr int L = L_, E = E_, TPL = L_ / E_;                        \
-    static constexpr int R0 = R0_, R1 = R1_, R2 = R2_, R3 = R3_;               \
-    static constexpr int NS = 1 + (R1_ > 1) + (R2_ > 1) + (R3_ > 1);          \
-    static_assert(R0_ * R1_ * R2_ * R3_ == L_, "plan product");                \
-    static_assert(E_ % R0_ == 0 && E_ % R1_ == 0 && E_ % R2_ == 0 && E_ % R3_ == 0, "plan E"); \
-  };
+template <int L_, int E_, int R0_, int R1_, int R2_, int R3_, int R4_ = 1> struct PlanT {
+  static constexpr int L = L_, E = E_, TPL = L_ / E_;
+  static constexpr int R0 = R0_, R1 = R1_, R2 = R2_, R3 = R3_, R4 = R4_;
+  static constexpr int NS = 1 + (R1_ > 1) + (R2_ > 1) + (R3_ > 1) + (R4_ > 1);
+  static_assert(R0_ * R1_ * R2_ * R3_ * R4_ == L_, "plan product");
+  static_assert(E_ % R0_ == 0 && E_ % R1_ == 0 && E_ % R2_ == 0 && E_ % R3_ == 0 && E_ % R4_ == 0,
+                "plan E");
+};
+
+// default plans: few stages (one LDS exchange where the registers allow)
+template <int L> struct PlanSel;
+#define MM_PLAN(L_, E_, R0_, R1_, R2_, R3_) \
+  template <> struct PlanSel<L_> { using type = PlanT<L_, E_, R0_, R1_, R2_, R3_>; };
 MM_PLAN(4, 4, 4, 1, 1, 1)
 MM_PLAN(6, 6, 6, 1, 1, 1)
 MM_PLAN(8, 8, 8, 1, 1, 1)
@@ -162,10 +166,34 @@ MM_PLAN(256, 16, 16, 16, 1, 1)
 MM_PLAN(384, 12, 4, 4, 4, 6)
 MM_PLAN(512, 16, 16, 16, 2, 1)
 #undef MM_PLAN
+template <int L> using Plan = typename PlanSel<L>::type;
+
+// lean plans for the fused z kernels, which keep several spectra of the
+// same line in registers: at most 12 points per thread (PlanLean) or, for
+// the six-spectrum elastic operation, 4 points per thread where the length
+// allows (PlanLean4)
+template <int L> struct PlanLeanSel { using type = Plan<L>; };
+template <> struct PlanLeanSel<128> { using type = PlanT<128, 8, 8, 4, 4, 1>; };
+template <> struct PlanLeanSel<256> { using type = PlanT<256, 8, 8, 8, 4, 1>; };
+template <> struct PlanLeanSel<512> { using type = PlanT<512, 8, 8, 8, 8, 1>; };
+template <int L> using PlanLean = typename PlanLeanSel<L>::type;
+
+template <int L> struct PlanLean4Sel { using type = PlanLean<L>; };
+template <> struct PlanLean4Sel<8> { using type = PlanT<8, 4, 4, 2, 1, 1>; };
+template <> struct PlanLean4Sel<32> { using type = PlanT<32, 4, 4, 4, 2, 1>; };
+template <> struct PlanLean4Sel<64> { using type = PlanT<64, 4, 4, 4, 4, 1>; };
+template <> struct PlanLean4Sel<128> { using type = PlanT<128, 4, 4, 4, 4, 2>; };
+template <> struct PlanLean4Sel<256> { using type = PlanT<256, 4, 4, 4, 4, 4>; };
+template <> struct PlanLean4Sel<512> { using type = PlanT<512, 8, 8, 8, 8, 1>; };
+template <int L> using PlanLean4 = typename PlanLean4Sel<L>::type;
 
 template <class P, int S> struct StageR {
-  static constexpr int R = S == 0 ? P::R0 : S == 1 ? P::R1 : S == 2 ? P::R2 : P::R3;
-  static constexpr int Ls = S == 0 ? 1 : S == 1 ? P::R0 : S == 2 ? P::R0 * P::R1 : P::R0 * P::R1 * P::R2;
+  static constexpr int R = S == 0 ? P::R0 : S == 1 ? P::R1 : S == 2 ? P::R2 : S == 3 ? P::R3 : P::R4;
+  static constexpr int Ls = S == 0 ? 1
+                            : S == 1 ? P::R0
+                            : S == 2 ? P::R0 * P::R1
+                            : S == 3 ? P::R0 * P::R1 * P::R2
+                                     : P::R0 * P::R1 * P::R2 * P::R3;
 };
 
 // One Stockham stage. `idx(e)` maps a line position to an LDS slot of this

@@ -83,7 +83,10 @@ k_x_r2c(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ tw
     const int b = t % B, r = t / B, y = y0 + b;
     if (y < ny) {
       const T hs = (T)0.5 * scale;
-      for (int kx = r; kx <= L; kx += TPL) {
+#pragma unroll
+      for (int i = 0; i <= E; ++i) {
+        const int kx = r + TPL * i;
+        if (kx > L) break;
         const int k1 = kx == L ? 0 : kx, k2 = kx == 0 ? 0 : L - kx;
         C2<T> zk = lds[b * LP + k1 + (k1 >> 4)];
         C2<T> zc = lds[b * LP + k2 + (k2 >> 4)];
@@ -109,10 +112,18 @@ k_x_c2r(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ tw
   const C2<T>* __restrict__ in = (const C2<T>*)bt.in[f];
   T* __restrict__ out = (T*)bt.out[f];
   {
+    // kx = r + TPL*i covers 0..L in E+1 rounds; unrolled so the loads overlap
     const int b = t % B, r = t / B, y = y0 + b;
-    for (int kx = r; kx <= L; kx += TPL) {
-      C2<T> x = (y < ny) ? in[((size_t)z * (L + 1) + kx) * ny + y] : mk<T>(0, 0);
-      lds[b * LP + kx + (kx >> 4)] = x;
+    C2<T> x[E + 1];
+#pragma unroll
+    for (int i = 0; i <= E; ++i) {
+      const int kx = r + TPL * i;
+      x[i] = (y < ny && kx <= L) ? in[((size_t)z * (L + 1) + kx) * ny + y] : mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i <= E; ++i) {
+      const int kx = r + TPL * i;
+      if (kx <= L) lds[b * LP + kx + (kx >> 4)] = x[i];
     }
   }
   __syncthreads();

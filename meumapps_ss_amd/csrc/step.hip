@@ -3,14 +3,20 @@
 // the mm_* whole-step entry points of include/meumapps_ss.h.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "fft3d.h"
 #include "mm_common.h"
 #include "step_kernels.h"
+#include "zfused.h"
 
 namespace mm {
+
+template <typename T, class Op>
+int launch_zfused(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int probe_kind, double bytes,
+                  int* nblk);
 
 // ============================================================== kernels
 
@@ -290,7 +296,7 @@ __device__ __forceinline__ void compliance_mix(const DevParams* __restrict__ P, 
 // ModElastic.f:248-327: C(x) = (sum P S)^-1, C' = CEla(:,:,2) - C(x),
 // x = cfac C' (eps + e_mean) + C(x) sum_v P_v eps0_v  with engineering
 // shears, evaluated as  cfac CEla2 b + C(x) (a - cfac b)  by one 6x6 solve.
-template <typename T, int V, int NPH>
+template <typename T, int V, int NPH, bool HOMOG>
 __global__ void __launch_bounds__(kBlock)
 k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ strain,
         double cfac, T* __restrict__ xpol, int* __restrict__ err, size_t n3) {
@@ -299,26 +305,48 @@ k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __r
     double Pv[V], ps = 0.0;
 #pragma unroll
     for (int v = 0; v < V; ++v) { Pv[v] = pfun(ld(phi, v * n3 + pt)); ps = ps + Pv[v]; }
-    double M[6][6];
-    compliance_mix<V, NPH>(P, Pv, ps, ps < 1.0, M);
-    double bv[6], r[6];
+    double bv[6], a[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       const double w = m < 3 ? 1.0 : 2.0;
       bv[m] = ld(strain, m * n3 + pt) + w * P->e_mean[m];
-      double a = 0.0;
-#pragma unroll
-      for (int v = 0; v < V; ++v) a += Pv[v] * P->tp[v * 6 + m];
-      r[m] = w * a - cfac * bv[m];
-    }
-    if (!solve6(M, r)) bad = true;
-    const double* C2m = P->cela + 36;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
       double s = 0.0;
 #pragma unroll
-      for (int n = 0; n < 6; ++n) s += C2m[n * 6 + m] * bv[n];
-      st(xpol, m * n3 + pt, cfac * s + r[m]);
+      for (int v = 0; v < V; ++v) s += Pv[v] * P->tp[v * 6 + m];
+      a[m] = w * s;
+    }
+    if constexpr (HOMOG) {
+      // one compliance S for every phase: the guarded mixture is S when
+      // sum P < 1 and (sum P) S otherwise, so C(x) = sc C with
+      // sc = 1 / max(1, sum P), C' = (1 - sc) C, and
+      // x = C (cfac (1 - sc) b + sc a)
+      const double sc = ps < 1.0 ? 1.0 : 1.0 / ps;
+      const double cb = cfac * (1.0 - sc);
+      double r[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) r[m] = cb * bv[m] + sc * a[m];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        double s = 0.0;
+#pragma unroll
+        for (int n = 0; n < 6; ++n) s += P->cela[n * 6 + m] * r[n];
+        st(xpol, m * n3 + pt, s);
+      }
+    } else {
+      double M[6][6];
+      compliance_mix<V, NPH>(P, Pv, ps, ps < 1.0, M);
+      double r[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) r[m] = a[m] - cfac * bv[m];
+      if (!solve6(M, r)) bad = true;
+      const double* C2m = P->cela + 36;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        double s = 0.0;
+#pragma unroll
+        for (int n = 0; n < 6; ++n) s += C2m[n * 6 + m] * bv[n];
+        st(xpol, m * n3 + pt, cfac * s + r[m]);
+      }
     }
   }
   if (bad) *err = MM_ERR_CINV_ITER;
@@ -326,7 +354,7 @@ k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __r
 
 // --------------------------------------------------------------- elener
 // ModElastic.f:493-691 fused with f_diff += el_grad (Multi_MEUMAPPS.f:880).
-template <typename T, int V, int NPH>
+template <typename T, int V, int NPH, bool HOMOG>
 __global__ void __launch_bounds__(kBlock)
 k_elener(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ strain,
          T* __restrict__ el_en, T* __restrict__ elgrad, T* __restrict__ fdiff,
@@ -344,51 +372,75 @@ k_elener(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __
 #pragma unroll
       for (int m = 0; m < 6; ++m) te[m] = te[m] + Pv[v] * P->tp[v * 6 + m];
     }
-    double sig[6];
+    double eps[6], sig[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       const double w = m < 3 ? 1.0 : 2.0;
-      sig[m] = ld(strain, m * n3 + pt) + w * P->e_mean[m] - w * te[m];  // eps
+      eps[m] = ld(strain, m * n3 + pt) + w * P->e_mean[m] - w * te[m];
     }
-    // the matrix term is unguarded here (ModElastic.f:597): with sum P > 1 the
-    // mixture can be indefinite, so this solve pivots
-    double M[6][6];
-    compliance_mix<V, NPH>(P, Pv, ps, true, M);
-    if (!solve_pivot<6>(M, sig)) bad = true;  // sig = C(x) eps
-    // el_mat = 1/2 sig . S_1 sig
-    double el_mat = 0.0;
+    double el_mat, el_ppt[NPH + 1];
+    if constexpr (HOMOG) {
+      // the unguarded mixture sum P S + (1 - sum P) S is S: sig = C eps, and
+      // every phase's energy density equals the matrix one
 #pragma unroll
-    for (int a = 0; a < 6; ++a) {
-      double em = 0.0;
+      for (int a = 0; a < 6; ++a) {
+        double s = 0.0;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) em = em + P->sela[c * 6 + a] * sig[c];
-      el_mat = el_mat + sig[a] * em;
+        for (int c = 0; c < 6; ++c) s += P->cela[c * 6 + a] * eps[c];
+        sig[a] = s;
+      }
+      double e2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) e2 += sig[a] * eps[a];
+      el_mat = 0.5 * e2;
+#pragma unroll
+      for (int q = 2; q <= NPH; ++q) el_ppt[q] = el_mat;
+    } else {
+      // the matrix term is unguarded here (ModElastic.f:597): with sum P > 1
+      // the mixture can be indefinite, so this solve pivots
+      double M[6][6];
+      compliance_mix<V, NPH>(P, Pv, ps, true, M);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) sig[m] = eps[m];
+      if (!solve_pivot<6>(M, sig)) bad = true;  // sig = C(x) eps
+      // el_mat = 1/2 sig . S_1 sig
+      el_mat = 0.0;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+        double em = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) em = em + P->sela[c * 6 + a] * sig[c];
+        el_mat = el_mat + sig[a] * em;
+      }
+      el_mat = 0.5 * el_mat;
+#pragma unroll
+      for (int q = 2; q <= NPH; ++q) {
+        double e = 0.0;
+        const double* S = P->sela + (q - 1) * 36;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+          double ep = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) ep = ep + S[c * 6 + a] * sig[c];
+          e = e + sig[a] * ep;
+        }
+        el_ppt[q] = 0.5 * e;
+      }
     }
-    el_mat = 0.5 * el_mat;
     double een = (1.0 - ps) * el_mat;
 #pragma unroll
     for (int q = 2; q <= NPH; ++q) {
-      double el_ppt = 0.0;
-      const double* S = P->sela + (q - 1) * 36;
-#pragma unroll
-      for (int a = 0; a < 6; ++a) {
-        double ep = 0.0;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) ep = ep + S[c * 6 + a] * sig[c];
-        el_ppt = el_ppt + sig[a] * ep;
-      }
-      el_ppt = 0.5 * el_ppt;
 #pragma unroll
       for (int v = 0; v < V; ++v)
         if (P->phase_of[v] == q) {
           double eg = 0.0;
           if (P->tsum_ok[v]) {
-            een = een + el_ppt * Pv[v];
+            een = een + el_ppt[q] * Pv[v];
             const double* t = P->tp + v * 6;
             const double d = (sig[0] * t[0] + sig[1] * t[1] + sig[2] * t[2] +
                               2.0 * (sig[3] * t[3] + sig[4] * t[4] + sig[5] * t[5]));
             const double pp = ppr(f[v]);
-            eg = pp * (el_mat - el_ppt) - d * pp;
+            eg = pp * (el_mat - el_ppt[q]) - d * pp;
           }
           st(elgrad, v * n3 + pt, eg);
           st(fdiff, v * n3 + pt, ld(fdiff, v * n3 + pt) + eg);
@@ -633,13 +685,15 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
   static void ppoly(dim3 g, hipStream_t s, const T* a, T* b, size_t n) {
     hipLaunchKernelGGL((k_ppoly<T, V>), g, dim3(kBlock), 0, s, a, b, n);
   }
+  template <bool H>
   static void polar(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, double cf, T* x,
                     int* err, size_t n) {
-    hipLaunchKernelGGL((k_polar<T, V, NPH>), g, dim3(kBlock), 0, s, P, a, b, cf, x, err, n);
+    hipLaunchKernelGGL((k_polar<T, V, NPH, H>), g, dim3(kBlock), 0, s, P, a, b, cf, x, err, n);
   }
+  template <bool H>
   static void elener(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, T* c, T* d, T* e,
                      double* p, int* err, size_t n) {
-    hipLaunchKernelGGL((k_elener<T, V, NPH>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, p, err, n);
+    hipLaunchKernelGGL((k_elener<T, V, NPH, H>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, p, err, n);
   }
   static void clamp(dim3 g, hipStream_t s, const DevParams* P, T* a, const T* b, double* p, size_t n) {
     hipLaunchKernelGGL((k_clamp<T, V, E>), g, dim3(kBlock), 0, s, P, a, b, p, n);
@@ -647,9 +701,11 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
   static void zeroth(dim3 g, hipStream_t s, const DevParams* P, const C2<T>* a, C2<T>* b, C2<T>* c, size_t n) {
     hipLaunchKernelGGL((k_ks_zeroth<T, V>), g, dim3(kBlock), 0, s, P, a, b, c, n);
   }
-  static Kern<T> table() {
+  static Kern<T> table(bool homog) {
     Kern<T> k;
-    k.chem = chem; k.ppoly = ppoly; k.polar = polar; k.elener = elener; k.clamp = clamp; k.zeroth = zeroth;
+    k.chem = chem; k.ppoly = ppoly; k.clamp = clamp; k.zeroth = zeroth;
+    k.polar = homog ? polar<true> : polar<false>;
+    k.elener = homog ? elener<true> : elener<false>;
     return k;
   }
 };
@@ -658,9 +714,9 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
 #define MM_COMBOS(X) \
   X(1, 2, 1) X(2, 2, 1) X(3, 2, 1) X(4, 2, 1) X(2, 3, 1) X(3, 3, 1) X(4, 3, 1) X(1, 2, 2) X(3, 2, 2)
 
-template <typename T> static bool kern_table(int V, int NPH, int E, Kern<T>& out) {
+template <typename T> static bool kern_table(int V, int NPH, int E, bool homog, Kern<T>& out) {
 #define X(v_, p_, e_) \
-  if (V == v_ && NPH == p_ && E == e_) { out = KernImpl<T, v_, p_, e_>::table(); return true; }
+  if (V == v_ && NPH == p_ && E == e_) { out = KernImpl<T, v_, p_, e_>::table(homog); return true; }
   MM_COMBOS(X)
 #undef X
   return false;
@@ -696,14 +752,16 @@ struct mm_ctx {
   Kern<double> k64;
   Kern<float> k32;
   Probe probe;
+  bool zfused = true;  // z passes with the k-space operations fused in (MM_ZFUSED=0: separate kernels)
   int last_niter = 0;
   double last_cfac = 1.0;
 };
 
 namespace {
 
-constexpr int kMaxPartSlots = 16;
-constexpr int kMaxBlocks = 2048;
+constexpr int kMaxPartSlots = 8;
+constexpr int kMaxBlocks = 2048;      // grid of the pointwise and k-space kernels
+constexpr int kMaxPartBlocks = 65536; // largest grid that writes partial sums
 // result slots in dres
 enum { R_FINT = 0, R_FCH, R_FW, R_EL, R_ERR0 /* 6 */, R_CLAMP = 10 /* E+4 */, R_N = 24 };
 
@@ -757,6 +815,11 @@ void host_params(const mm_params& p, int prec, DevParams& d) {
       d.zrc[i0 - 1] = GP(e, ph) * GP(i1, ph) - GP(e, ph + 1) * GP(i1, ph + 1);
     }
   }
+  d.homog = 1;
+  for (int ph = 1; ph < p.numph; ++ph)
+    if (std::memcmp(p.cela + ph * 36, p.cela, 36 * sizeof(double)) != 0 ||
+        std::memcmp(p.sela + ph * 36, p.sela, 36 * sizeof(double)) != 0)
+      d.homog = 0;
   (void)prec;
 }
 
@@ -803,6 +866,25 @@ template <typename T> int inv(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf
   return 0;
 }
 
+// x and y passes only (the z pass is fused with a k-space operation)
+template <typename T> int fwd_xy(mm_ctx* c, const T* const* in, C2<T>* const* out, int nf) {
+  const T scale = (T)(1.0 / (double)(float)c->n3);
+  for (int f0 = 0; f0 < nf; f0 += kMaxBatch) {
+    const int g = std::min(kMaxBatch, nf - f0);
+    MM_TRY(c->fft.x_r2c<T>(in + f0, (void* const*)(out + f0), g, c->s, scale));
+    MM_TRY(c->fft.y_pass<T>((void* const*)(out + f0), (void* const*)(out + f0), g, -1, c->s));
+  }
+  return 0;
+}
+template <typename T> int inv_yx(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf) {
+  for (int f0 = 0; f0 < nf; f0 += kMaxBatch) {
+    const int g = std::min(kMaxBatch, nf - f0);
+    MM_TRY(c->fft.y_pass<T>((void* const*)(sp + f0), (void* const*)(sp + f0), g, +1, c->s));
+    MM_TRY(c->fft.x_c2r<T>((void* const*)(sp + f0), out + f0, g, c->s));
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------------ eldis
 template <typename T> int eldis(mm_ctx* c) {
   const size_t n3 = c->n3, nc = c->nc;
@@ -835,15 +917,29 @@ template <typename T> int eldis(mm_ctx* c) {
     K.polar(gp, c->s, c->dp, phi, strain, cfac, xpol, c->derr, n3);
     c->probe.end(MM_PROBE_POLAR, c->s, (double)n3 * sizeof(T) * (c->V + 12));
     MM_HIP(hipGetLastError());
-    MM_TRY(fwd<T>(c, xpol, n3, work, 6));
-    c->probe.begin(MM_PROBE_KS_ELASTIC, c->s);
-    hipLaunchKernelGGL(k_ks_elastic<T>, gk, dim3(kBlock), 0, c->s, c->dp, work, uhat, c->part, nc);
-    c->probe.end(MM_PROBE_KS_ELASTIC, c->s, (double)nc * 2 * sizeof(T) * 18);
-    MM_HIP(hipGetLastError());
-    MM_TRY(finalize(c, c->nblk_k, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
+    if (c->zfused) {
+      const T* xin[6];
+      for (int m = 0; m < 6; ++m) xin[m] = xpol + (size_t)m * n3;
+      MM_TRY(fwd_xy<T>(c, xin, wsp, 6));
+      OpElastic<T> op{c->dp, work, uhat, c->part, nc};
+      int nblk = 0;
+      MM_TRY(launch_zfused<T>(c->fft, op, 1, c->s, MM_PROBE_ZF_ELASTIC, (double)nc * 2 * sizeof(T) * 18, &nblk));
+      MM_TRY(finalize(c, nblk, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
+    } else {
+      MM_TRY(fwd<T>(c, xpol, n3, work, 6));
+      c->probe.begin(MM_PROBE_KS_ELASTIC, c->s);
+      hipLaunchKernelGGL(k_ks_elastic<T>, gk, dim3(kBlock), 0, c->s, c->dp, work, uhat, c->part, nc);
+      c->probe.end(MM_PROBE_KS_ELASTIC, c->s, (double)nc * 2 * sizeof(T) * 18);
+      MM_HIP(hipGetLastError());
+      MM_TRY(finalize(c, c->nblk_k, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
+    }
     MM_HIP(hipMemcpyAsync(c->hres + R_ERR0, c->dres + R_ERR0, 6 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     MM_HIP(hipEventRecord(c->ev, c->s));
-    MM_TRY(inv<T>(c, wsp, wre, 6));
+    if (c->zfused) {
+      MM_TRY(inv_yx<T>(c, wsp, wre, 6));
+    } else {
+      MM_TRY(inv<T>(c, wsp, wre, 6));
+    }
     MM_HIP(hipEventSynchronize(c->ev));
     const double* err = c->hres + R_ERR0;
     // convergence / backtracking control, ModElastic.f:435-481
@@ -893,12 +989,24 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
   if (c->step <= c->prm.noise_step) return MM_ERR_ARG;  // Langevin noise is host-side, not supported here
   MM_HIP(hipMemsetAsync(c->derr, 0, sizeof(int), c->s));
   // 1. phi_hat (Multi_MEUMAPPS.f:577-578); kept for the Allen-Cahn update
-  MM_TRY(fwd<T>(c, phi, n3, sphi, V));
+  if (c->zfused) {
+    const T* pin[MM_MAX_VAR];
+    C2<T>* pout[MM_MAX_VAR];
+    for (int v = 0; v < V; ++v) { pin[v] = phi + (size_t)v * n3; pout[v] = sphi + (size_t)v * nc; }
+    MM_TRY(fwd_xy<T>(c, pin, pout, V));
+  } else {
+    MM_TRY(fwd<T>(c, phi, n3, sphi, V));
+  }
   // 2. gradient block (:581-613) in groups of up to 4 variants
   const int G = std::max(1, std::min({V, c->nwork / 4, 4}));
   for (int v0 = 0; v0 < V; v0 += G) {
     const int g = std::min(G, V - v0);
-    hipLaunchKernelGGL(k_ks_grad<T>, dim3(c->nblk_k, g), dim3(kBlock), 0, c->s, c->dp, sphi, work, v0, nc);
+    if (c->zfused) {
+      OpGrad<T> op{c->dp, sphi, work, nc, v0};
+      MM_TRY(launch_zfused<T>(c->fft, op, g, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 6 * g, nullptr));
+    } else {
+      hipLaunchKernelGGL(k_ks_grad<T>, dim3(c->nblk_k, g), dim3(kBlock), 0, c->s, c->dp, sphi, work, v0, nc);
+    }
     MM_HIP(hipGetLastError());
     C2<T>* sp[16];
     T* rp[16];
@@ -908,7 +1016,11 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
       sp[4 * j + 3] = work + (size_t)(4 * j + 3) * nc;
       rp[4 * j + 3] = (T*)c->kdel + (size_t)v * n3;
     }
-    MM_TRY(inv<T>(c, sp, rp, 4 * g));
+    if (c->zfused) {
+      MM_TRY(inv_yx<T>(c, sp, rp, 4 * g));
+    } else {
+      MM_TRY(inv<T>(c, sp, rp, 4 * g));
+    }
   }
   // 3. chemistry (:634-845)
   c->probe.begin(MM_PROBE_CHEM, c->s);
@@ -935,24 +1047,48 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
                         (T*)c->flux + (size_t)(3 * e + 1) * n3, (T*)c->flux + (size_t)(3 * e + 2) * n3};
       void* op[4] = {work + (size_t)(4 * e) * nc, work + (size_t)(4 * e + 1) * nc, work + (size_t)(4 * e + 2) * nc,
                      work + (size_t)(4 * e + 3) * nc};
-      MM_TRY(c->fft.forward<T>(ip, op, 4, c->s, scale));
+      if (c->zfused) {
+        MM_TRY(fwd_xy<T>(c, ip, (C2<T>* const*)op, 4));
+      } else {
+        MM_TRY(c->fft.forward<T>(ip, op, 4, c->s, scale));
+      }
     }
-    hipLaunchKernelGGL(k_ks_ch<T>, dim3(c->nblk_k, E), dim3(kBlock), 0, c->s, c->dp, work, nc);
+    if (c->zfused) {
+      OpCH<T> opch{c->dp, work, nc};
+      MM_TRY(launch_zfused<T>(c->fft, opch, E, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 5 * E, nullptr));
+    } else {
+      hipLaunchKernelGGL(k_ks_ch<T>, dim3(c->nblk_k, E), dim3(kBlock), 0, c->s, c->dp, work, nc);
+    }
     MM_HIP(hipGetLastError());
     C2<T>* sp[MM_MAX_EL];
     T* rp[MM_MAX_EL];
     for (int e = 0; e < E; ++e) { sp[e] = work + (size_t)(4 * e) * nc; rp[e] = (T*)c->cons + (size_t)e * n3; }
-    MM_TRY(inv<T>(c, sp, rp, E));
+    if (c->zfused) {
+      MM_TRY(inv_yx<T>(c, sp, rp, E));
+    } else {
+      MM_TRY(inv<T>(c, sp, rp, E));
+    }
   }
   // 6. Allen-Cahn (:955-1001)
   {
-    MM_TRY(fwd<T>(c, (T*)c->fdiff, n3, work, V));
-    hipLaunchKernelGGL(k_ks_ac<T>, dim3(c->nblk_k, V), dim3(kBlock), 0, c->s, c->dp, sphi, work, nc);
-    MM_HIP(hipGetLastError());
     C2<T>* sp[MM_MAX_VAR];
     T* rp[MM_MAX_VAR];
-    for (int v = 0; v < V; ++v) { sp[v] = sphi + (size_t)v * nc; rp[v] = phi + (size_t)v * n3; }
-    MM_TRY(inv<T>(c, sp, rp, V));
+    if (c->zfused) {
+      const T* fin[MM_MAX_VAR];
+      C2<T>* fout[MM_MAX_VAR];
+      for (int v = 0; v < V; ++v) { fin[v] = (T*)c->fdiff + (size_t)v * n3; fout[v] = work + (size_t)v * nc; }
+      MM_TRY(fwd_xy<T>(c, fin, fout, V));
+      OpAC<T> opac{c->dp, work, sphi, nc};
+      MM_TRY(launch_zfused<T>(c->fft, opac, V, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 3 * V, nullptr));
+      for (int v = 0; v < V; ++v) { sp[v] = work + (size_t)v * nc; rp[v] = phi + (size_t)v * n3; }
+      MM_TRY(inv_yx<T>(c, sp, rp, V));
+    } else {
+      MM_TRY(fwd<T>(c, (T*)c->fdiff, n3, work, V));
+      hipLaunchKernelGGL(k_ks_ac<T>, dim3(c->nblk_k, V), dim3(kBlock), 0, c->s, c->dp, sphi, work, nc);
+      MM_HIP(hipGetLastError());
+      for (int v = 0; v < V; ++v) { sp[v] = sphi + (size_t)v * nc; rp[v] = phi + (size_t)v * n3; }
+      MM_TRY(inv<T>(c, sp, rp, V));
+    }
   }
   // 7. clamp and check-loop sums (:1008-1065)
   K.clamp(gp, c->s, c->dp, phi, (T*)c->cons, c->part, n3);
@@ -1158,15 +1294,19 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   c->prec = precision;
   c->V = p->vartot; c->E = p->numel; c->NPH = p->numph;
   c->step = p->step;
-  bool ok = precision == MM_FP64 ? kern_table<double>(c->V, c->NPH, c->E, c->k64)
-                                 : kern_table<float>(c->V, c->NPH, c->E, c->k32);
+  host_params(*p, precision, c->hp);
+  bool ok = precision == MM_FP64 ? kern_table<double>(c->V, c->NPH, c->E, c->hp.homog != 0, c->k64)
+                                 : kern_table<float>(c->V, c->NPH, c->E, c->hp.homog != 0, c->k32);
   int rc = ok ? c->fft.init(p->nx, p->ny, p->nz, precision) : MM_ERR_ARG;
   c->fft.probe = &c->probe;
+  {
+    const char* zf = std::getenv("MM_ZFUSED");
+    c->zfused = !(zf && zf[0] == '0');
+  }
   if (rc) { free_all(c); delete c; return rc; }
   c->n3 = c->fft.nreal();
   c->nc = c->fft.ncplx();
   c->esz = precision == MM_FP64 ? 8 : 4;
-  host_params(*p, precision, c->hp);
   const size_t n3 = c->n3, nc = c->nc, rb = c->esz, cb = 2 * c->esz;
   const int V = c->V, E = c->E;
   c->nwork = std::max({6 + V, 4 * std::min(V, 4), 4 * E, 6});
@@ -1187,7 +1327,7 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   rc = rc ? rc : dalloc(&c->sphi, V * nc * cb);
   rc = rc ? rc : dalloc(&c->suhat, 3 * nc * cb);
   rc = rc ? rc : dalloc(&c->swork, c->nwork * nc * cb);
-  rc = rc ? rc : dalloc((void**)&c->part, (size_t)kMaxPartSlots * kMaxBlocks * sizeof(double));
+  rc = rc ? rc : dalloc((void**)&c->part, (size_t)kMaxPartSlots * kMaxPartBlocks * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->dres, 64 * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->derr, 16);
   rc = rc ? rc : dalloc((void**)&c->dp, sizeof(DevParams));
@@ -1280,7 +1420,7 @@ extern "C" int mm_field_to_host(mm_ctx* c, int32_t which, double* host) {
 }
 
 extern "C" int mm_probe(mm_ctx* c, int32_t kind) {
-  if (!c || kind < -1 || kind > MM_PROBE_KS_ELASTIC) return MM_ERR_ARG;
+  if (!c || kind < -1 || kind > MM_PROBE_ZF_OTHER) return MM_ERR_ARG;
   MM_HIP(hipStreamSynchronize(c->s));
   MM_TRY(c->probe.collect());
   c->probe.reset();

@@ -36,6 +36,7 @@ struct DevParams {
   double zc[16 * 16];  // constant rows of Zintf, row-major [i][j], 0-based
   double zrc[16];      // constant entries of Zr
   double cbar[MM_MAX_EL];  // MIXED: mean composition held in FP64 (0 in FP64 mode)
+  int homog;               // every phase has the same CEla and SEla (bitwise)
 };
 
 constexpr int kBlock = 256;
@@ -51,8 +52,8 @@ template <int OP> __device__ __forceinline__ double rop(double a, double b) {
   else return b > a ? b : a;
 }
 
-// Block-wide reduction in a fixed order; every thread gets the result.
-template <int OP> __device__ __forceinline__ double block_reduce(double v, double* sh) {
+// Block-wide reduction in a fixed order over NW waves; every thread gets the result.
+template <int OP, int NW = kWaves> __device__ __forceinline__ double block_reduce(double v, double* sh) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = rop<OP>(v, __shfl_down(v, off, 64));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -60,15 +61,15 @@ template <int OP> __device__ __forceinline__ double block_reduce(double v, doubl
   __syncthreads();
   double r = sh[0];
 #pragma unroll
-  for (int i = 1; i < kWaves; ++i) r = rop<OP>(r, sh[i]);
+  for (int i = 1; i < NW; ++i) r = rop<OP>(r, sh[i]);
   __syncthreads();
   return r;
 }
 
 // part[k*nblk + blockIdx.x] = block result of slot k
-template <int OP>
+template <int OP, int NW = kWaves>
 __device__ __forceinline__ void block_store(double v, double* part, int k, double* sh) {
-  double r = block_reduce<OP>(v, sh);
+  double r = block_reduce<OP, NW>(v, sh);
   if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = r;
 }
 
@@ -94,17 +95,22 @@ struct KVec {
   int kx, ky, kz;
 };
 
+__device__ __forceinline__ KVec kvec3(const DevParams* __restrict__ P, int kx, int ky, int kz) {
+  KVec k;
+  k.kx = kx;
+  k.ky = ky;
+  k.kz = kz;
+  kaxis(kx, P->nx, P->unit[0], k.kf[0], k.kq[0]);
+  kaxis(ky, P->ny, P->unit[1], k.kf[1], k.kq[1]);
+  kaxis(kz, P->nz, P->unit[2], k.kf[2], k.kq[2]);
+  return k;
+}
+
 // spectrum index i -> wave vector; layout [kz][kx][ky]
 __device__ __forceinline__ KVec kvec(const DevParams* __restrict__ P, size_t i) {
-  KVec k;
-  k.ky = (int)(i % (size_t)P->ny);
-  size_t r = i / (size_t)P->ny;
-  k.kx = (int)(r % (size_t)P->H);
-  k.kz = (int)(r / (size_t)P->H);
-  kaxis(k.kx, P->nx, P->unit[0], k.kf[0], k.kq[0]);
-  kaxis(k.ky, P->ny, P->unit[1], k.kf[1], k.kq[1]);
-  kaxis(k.kz, P->nz, P->unit[2], k.kf[2], k.kq[2]);
-  return k;
+  const int ky = (int)(i % (size_t)P->ny);
+  const size_t r = i / (size_t)P->ny;
+  return kvec3(P, (int)(r % (size_t)P->H), ky, (int)(r / (size_t)P->H));
 }
 
 // sum G_ij k_i k_j with kf_sq on the diagonal (Multi_MEUMAPPS.f:596-600)

@@ -1,0 +1,383 @@
+// zfused.h -- z pass with the k-space operation of the step fused between the
+// forward and the inverse transform of every column.
+//
+// The x and y passes leave spectra partially transformed; one workgroup owns
+// B whole z-columns (kx, ky), runs the forward z transform of every input
+// spectrum of its columns, applies the pointwise k-space operation with all
+// inputs of a wave vector in registers, and runs the inverse z transform of
+// every output.  Each input and output crosses HBM once instead of three
+// times (z forward, k-space kernel, z inverse).
+//
+//   OpGrad     gradient block      Multi_MEUMAPPS.f:577-613   1 in  -> 4 out (+ full phi_hat)
+//   OpElastic  elastic iteration   ModElastic.f:329-433       6 in  -> 6 out (+ u_hat, sums)
+//   OpCH       Cahn-Hilliard       Multi_MEUMAPPS.f:912-947   4 in  -> 1 out
+//   OpAC       Allen-Cahn          Multi_MEUMAPPS.f:958-997   1 in  -> 1 out (reads phi_hat)
+#pragma once
+#include "fft_pass.h"
+#include "step_kernels.h"
+
+namespace mm {
+
+// Workgroup size: up to 512 threads; plans with more than 8 points per
+// thread (radix-3 lengths) use 256 so that their registers fit.
+template <typename T, class P, int MAXT> struct TileZF {
+  static constexpr int TPL = P::TPL;
+  static constexpr int MT = P::E > 8 ? cmin(MAXT, 256) : MAXT;
+  // minimum waves per SIMD requested from the register allocator
+  template <int MINW> static constexpr int minw() { return P::E <= 4 ? MINW : (MINW > 2 ? 2 : MINW); }
+  static constexpr int B = pow2floor(cmin(MT / TPL, kLdsBudget / (P::L * (int)sizeof(C2<T>))));
+  static constexpr int THREADS = B * TPL;
+  static constexpr int LDS = B * P::L;
+  static constexpr int NW = (THREADS + 63) / 64;
+};
+
+// per-thread view of one column: kx, ky fixed, kz = tl + TPL*q
+struct ColK {
+  double kf[2], kq[2];
+  int kx, ky;
+};
+__device__ __forceinline__ ColK colk(const DevParams* __restrict__ P, int col) {
+  ColK c;
+  c.kx = col / P->ny;
+  c.ky = col % P->ny;
+  kaxis(c.kx, P->nx, P->unit[0], c.kf[0], c.kq[0]);
+  kaxis(c.ky, P->ny, P->unit[1], c.kf[1], c.kq[1]);
+  return c;
+}
+__device__ __forceinline__ KVec kfull(const DevParams* __restrict__ P, const ColK& c, int kz) {
+  KVec k;
+  k.kx = c.kx; k.ky = c.ky; k.kz = kz;
+  k.kf[0] = c.kf[0]; k.kq[0] = c.kq[0];
+  k.kf[1] = c.kf[1]; k.kq[1] = c.kq[1];
+  kaxis(kz, P->nz, P->unit[2], k.kf[2], k.kq[2]);
+  return k;
+}
+
+template <int L, bool FOUR> struct ZPlan { using type = PlanLean<L>; };
+template <int L> struct ZPlan<L, true> { using type = PlanLean4<L>; };
+
+// The twiddle pointer and the thread's line index are laundered through an
+// empty asm statement before every transform: otherwise the compiler hoists
+// all loop-invariant twiddle loads of the NIN + NOUT transforms to the kernel
+// entry and keeps them live, which costs over 60 VGPRs.
+template <typename T, int L, class Op>
+__global__ void __launch_bounds__((TileZF<T, typename ZPlan<L, Op::LEAN4>::type, Op::MAXT>::THREADS),
+                                  (TileZF<T, typename ZPlan<L, Op::LEAN4>::type, Op::MAXT>::template minw<Op::MINW>()))
+k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
+  using P = typename ZPlan<L, Op::LEAN4>::type;
+  using TL = TileZF<T, P, Op::MAXT>;
+  constexpr int TPL = P::TPL, E = P::E, B = TL::B;
+  __shared__ C2<T> lds[TL::LDS];
+  const int t = threadIdx.x, b = t % B, tl = t / B, col = blockIdx.x * B + b, g = blockIdx.y;
+  const bool valid = col < ncols;
+  const int colc = valid ? col : 0;
+  const ColK ck = colk(op.P, colc);
+  IdxElemMajor idx{b, B};
+  typename Op::template State<E> st;
+#pragma unroll 1
+  for (int f = 0; f < Op::NIN; ++f) {
+    C2<T> v[E];
+    const C2<T>* in = op.input(g, f) + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = valid ? in[(size_t)(tl + TPL * q) * ncols] : mk<T>(0, 0);
+    if (f > 0) __syncthreads();
+    const C2<T>* tw = twL;
+    int tlx = tl;
+    asm volatile("" : "+s"(tw), "+v"(tlx));
+    line_fft<T, -1, P>(v, tlx, lds, idx, tw);
+    op.template absorb<E>(st, g, f, v, ck, tlx, TPL);
+  }
+  op.template finish<E>(st, g, ck, colc, tl, TPL, ncols, valid);
+#pragma unroll 1
+  for (int o = 0; o < Op::NOUT; ++o) {
+    C2<T> v[E];
+    const C2<T>* tw = twL;
+    int tlx = tl;
+    asm volatile("" : "+s"(tw), "+v"(tlx));
+    op.template emit<E>(st, g, o, v, ck, tlx, TPL);
+    __syncthreads();
+    line_fft<T, +1, P>(v, tlx, lds, idx, tw);
+    C2<T>* out = op.output(g, o) + col;
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) out[(size_t)(tl + TPL * q) * ncols] = v[q];
+    }
+  }
+  op.template reduce<TL::NW>(st);
+}
+
+// ----------------------------------------------------------- gradient block
+// in: phi_hat after the x and y passes (sphi[v]); the z-transformed phi_hat is
+// written back in place for the Allen-Cahn update.  out[4j+d] = phi_hat * i kf_d,
+// out[4j+3] = phi_hat * sum G kk.
+template <typename T> struct OpGrad {
+  static constexpr int NIN = 1, NOUT = 4, MAXT = 512, MINW = 4;
+  static constexpr bool LEAN4 = false;
+  const DevParams* P;
+  C2<T>* sphi;
+  C2<T>* work;
+  size_t nc;
+  int v0;
+  template <int E> struct State { C2<T> s[E]; };
+  __device__ const C2<T>* input(int g, int) const { return sphi + (size_t)(v0 + g) * nc; }
+  __device__ C2<T>* output(int g, int o) const { return work + (size_t)(4 * g + o) * nc; }
+  template <int E>
+  __device__ void absorb(State<E>& st, int, int, const C2<T> (&v)[E], const ColK&, int, int) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) st.s[q] = v[q];
+  }
+  template <int E>
+  __device__ void finish(State<E>& st, int g, const ColK&, int col, int tl, int TPL, int ncols, bool valid) const {
+    if (!valid) return;
+    C2<T>* dst = sphi + (size_t)(v0 + g) * nc + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) dst[(size_t)(tl + TPL * q) * ncols] = st.s[q];
+  }
+  template <int O, int E>
+  __device__ void emit_o(const State<E>& st, int g, C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    const int var = v0 + g;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const KVec k = kfull(P, ck, tl + TPL * q);
+      const double sr = (double)st.s[q].x, si = (double)st.s[q].y;
+      if constexpr (O < 3) {
+        const double a = k.kf[O];
+        v[q] = mk<T>((T)(-si * a), (T)(sr * a));
+      } else {
+        const double gk = kgk(P, var, k);
+        v[q] = mk<T>((T)(sr * gk), (T)(si * gk));
+      }
+    }
+  }
+  template <int E>
+  __device__ void emit(const State<E>& st, int g, int o, C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    switch (o) {
+      case 0: emit_o<0, E>(st, g, v, ck, tl, TPL); break;
+      case 1: emit_o<1, E>(st, g, v, ck, tl, TPL); break;
+      case 2: emit_o<2, E>(st, g, v, ck, tl, TPL); break;
+      default: emit_o<3, E>(st, g, v, ck, tl, TPL); break;
+    }
+  }
+  template <int NW, class S> __device__ void reduce(const S&) const {}
+};
+
+// --------------------------------------------------------- elastic iteration
+// in: stress-polarisation spectra x_1..x_6 after x and y (work[0..5]);
+// R = kf . x_hat, u = Q^-1 R (u(0) = 0); Parseval sums of |u - u_old|^2 and
+// |u_old|^2 over the half spectrum; out: engineering-strain spectra (in place).
+template <typename T> struct OpElastic {
+  static constexpr int NIN = 6, NOUT = 6, MAXT = 512, MINW = 1;
+  static constexpr bool LEAN4 = true;
+  const DevParams* P;
+  C2<T>* work;
+  C2<T>* uhat;
+  double* part;
+  size_t nc;
+  template <int E> struct State {
+    double rr[3][E], ri[3][E];  // R, then u
+    double acc[6];
+  };
+  __device__ const C2<T>* input(int, int f) const { return work + (size_t)f * nc; }
+  __device__ C2<T>* output(int, int o) const { return work + (size_t)o * nc; }
+  // x_f contributes  i kf_j x_f  to R_i for every (i,j) with voigt(i,j) = f:
+  // R(1) = x1 kf1 + x6 kf2 + x5 kf3; R(2) = x6 kf1 + x2 kf2 + x4 kf3; R(3) = x5 kf1 + x4 kf2 + x3 kf3
+  template <int F, int E>
+  __device__ void absorb_f(State<E>& st, const C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    constexpr int ia[6] = {0, 1, 2, 1, 0, 0}, ja[6] = {0, 1, 2, 2, 2, 1};
+    constexpr int ib[6] = {-1, -1, -1, 2, 2, 1}, jb[6] = {-1, -1, -1, 1, 0, 0};
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      if constexpr (F == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { st.rr[i][q] = 0.0; st.ri[i][q] = 0.0; }
+      }
+      double kf[3], kq2;
+      kf[0] = ck.kf[0];
+      kf[1] = ck.kf[1];
+      kaxis(tl + TPL * q, P->nz, P->unit[2], kf[2], kq2);
+      const double xr = (double)v[q].x, xi = (double)v[q].y;
+      // i*a*(xr + i xi) = (-a xi, a xr)
+      st.rr[ia[F]][q] += -kf[ja[F]] * xi;
+      st.ri[ia[F]][q] += kf[ja[F]] * xr;
+      if constexpr (ib[F] >= 0) {
+        st.rr[ib[F]][q] += -kf[jb[F]] * xi;
+        st.ri[ib[F]][q] += kf[jb[F]] * xr;
+      }
+    }
+  }
+  template <int E>
+  __device__ void absorb(State<E>& st, int, int f, const C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    switch (f) {
+      case 0: absorb_f<0, E>(st, v, ck, tl, TPL); break;
+      case 1: absorb_f<1, E>(st, v, ck, tl, TPL); break;
+      case 2: absorb_f<2, E>(st, v, ck, tl, TPL); break;
+      case 3: absorb_f<3, E>(st, v, ck, tl, TPL); break;
+      case 4: absorb_f<4, E>(st, v, ck, tl, TPL); break;
+      default: absorb_f<5, E>(st, v, ck, tl, TPL); break;
+    }
+  }
+  template <int E>
+  __device__ void finish(State<E>& st, int, const ColK& ck, int col, int tl, int TPL, int ncols, bool valid) const {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) st.acc[m] = 0.0;
+    const double w = (ck.kx == 0 || 2 * ck.kx == P->nx) ? 1.0 : 2.0;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int kz = tl + TPL * q;
+      const KVec k = kfull(P, ck, kz);
+      Cplx3 u;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) { u.re[i] = st.rr[i][q]; u.im[i] = st.ri[i][q]; }
+      if (k.kx == 0 && k.ky == 0 && kz == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { u.re[i] = 0.0; u.im[i] = 0.0; }
+      } else {
+        acoustic_solve(P, k, u);
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) { st.rr[i][q] = u.re[i]; st.ri[i][q] = u.im[i]; }
+      if (valid) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          C2<T>* uh = uhat + (size_t)i * nc + (size_t)kz * ncols + col;
+          const C2<T> uo = *uh;
+          const double dr = u.re[i] - (double)uo.x, di = u.im[i] - (double)uo.y;
+          st.acc[2 * i] += w * (dr * dr + di * di);
+          st.acc[2 * i + 1] += w * ((double)uo.x * (double)uo.x + (double)uo.y * (double)uo.y);
+          *uh = mk<T>((T)u.re[i], (T)u.im[i]);
+        }
+      }
+    }
+  }
+  // e11, e22, e33, g23, g13, g12 = i kf_A u_B (+ i kf_A2 u_B2)
+  template <int O, int E>
+  __device__ void emit_o(const State<E>& st, C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    constexpr int A[6] = {0, 1, 2, 1, 0, 0}, Bi[6] = {0, 1, 2, 2, 2, 1};
+    constexpr int A2[6] = {-1, -1, -1, 2, 2, 1}, B2[6] = {-1, -1, -1, 1, 0, 0};
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      double kf[3], kq2;
+      kf[0] = ck.kf[0];
+      kf[1] = ck.kf[1];
+      kaxis(tl + TPL * q, P->nz, P->unit[2], kf[2], kq2);
+      double re = -kf[A[O]] * st.ri[Bi[O]][q], im = kf[A[O]] * st.rr[Bi[O]][q];
+      if constexpr (A2[O] >= 0) {
+        re += -kf[A2[O]] * st.ri[B2[O]][q];
+        im += kf[A2[O]] * st.rr[B2[O]][q];
+      }
+      v[q] = mk<T>((T)re, (T)im);
+    }
+  }
+  template <int E>
+  __device__ void emit(const State<E>& st, int, int o, C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    switch (o) {
+      case 0: emit_o<0, E>(st, v, ck, tl, TPL); break;
+      case 1: emit_o<1, E>(st, v, ck, tl, TPL); break;
+      case 2: emit_o<2, E>(st, v, ck, tl, TPL); break;
+      case 3: emit_o<3, E>(st, v, ck, tl, TPL); break;
+      case 4: emit_o<4, E>(st, v, ck, tl, TPL); break;
+      default: emit_o<5, E>(st, v, ck, tl, TPL); break;
+    }
+  }
+  template <int NW, class S> __device__ void reduce(const S& st) const {
+    __shared__ double sh[NW];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) block_store<0, NW>(st.acc[m], part, m, sh);
+  }
+};
+
+// ------------------------------------------------------------ Cahn-Hilliard
+// in: work[4e] = c, work[4e+1..3] = Fx, Fy, Fz after x and y; out: c_hat new.
+template <typename T> struct OpCH {
+  static constexpr int NIN = 4, NOUT = 1, MAXT = 512, MINW = 4;
+  static constexpr bool LEAN4 = true;
+  const DevParams* P;
+  C2<T>* work;
+  size_t nc;
+  template <int E> struct State { double cr[E], ci[E], tr[E], ti[E]; };
+  __device__ const C2<T>* input(int g, int f) const { return work + (size_t)(4 * g + f) * nc; }
+  __device__ C2<T>* output(int g, int) const { return work + (size_t)(4 * g) * nc; }
+  template <int F, int E>
+  __device__ void absorb_f(State<E>& st, const C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      if constexpr (F == 0) {
+        st.cr[q] = (double)v[q].x;
+        st.ci[q] = (double)v[q].y;
+        st.tr[q] = 0.0;
+        st.ti[q] = 0.0;
+      } else {
+        double a, kq2;
+        if constexpr (F < 3) a = ck.kf[F - 1];
+        else kaxis(tl + TPL * q, P->nz, P->unit[2], a, kq2);
+        st.tr[q] += -a * (double)v[q].y;
+        st.ti[q] += a * (double)v[q].x;
+      }
+    }
+  }
+  template <int E>
+  __device__ void absorb(State<E>& st, int, int f, const C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    switch (f) {
+      case 0: absorb_f<0, E>(st, v, ck, tl, TPL); break;
+      case 1: absorb_f<1, E>(st, v, ck, tl, TPL); break;
+      case 2: absorb_f<2, E>(st, v, ck, tl, TPL); break;
+      default: absorb_f<3, E>(st, v, ck, tl, TPL); break;
+    }
+  }
+  template <int E>
+  __device__ void finish(State<E>&, int, const ColK&, int, int, int, int, bool) const {}
+  template <int E>
+  __device__ void emit(const State<E>& st, int g, int, C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    const double dtm = P->dt * P->cmobs[g];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const KVec k = kfull(P, ck, tl + TPL * q);
+      const double den = 1.0 - dtm * (k.kq[0] + k.kq[1] + k.kq[2]);
+      v[q] = mk<T>((T)((st.cr[q] + dtm * st.tr[q]) / den), (T)((st.ci[q] + dtm * st.ti[q]) / den));
+    }
+  }
+  template <int NW, class S> __device__ void reduce(const S&) const {}
+};
+
+// -------------------------------------------------------------- Allen-Cahn
+// in: work[v] = f_diff after x and y; reads the full phi_hat (sphi[v]);
+// out (in place): (phi_hat - dt L f_hat) / (1 - dt L sum G kk).
+template <typename T> struct OpAC {
+  static constexpr int NIN = 1, NOUT = 1, MAXT = 512, MINW = 4;
+  static constexpr bool LEAN4 = false;
+  const DevParams* P;
+  C2<T>* work;
+  const C2<T>* sphi;
+  size_t nc;
+  template <int E> struct State { double fr[E], fi[E], pr[E], pi[E]; };
+  __device__ const C2<T>* input(int g, int) const { return work + (size_t)g * nc; }
+  __device__ C2<T>* output(int g, int) const { return work + (size_t)g * nc; }
+  template <int E>
+  __device__ void absorb(State<E>& st, int, int, const C2<T> (&v)[E], const ColK&, int, int) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) { st.fr[q] = (double)v[q].x; st.fi[q] = (double)v[q].y; }
+  }
+  template <int E>
+  __device__ void finish(State<E>& st, int g, const ColK&, int col, int tl, int TPL, int ncols, bool valid) const {
+    const C2<T>* src = sphi + (size_t)g * nc + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const C2<T> s = valid ? src[(size_t)(tl + TPL * q) * ncols] : mk<T>(0, 0);
+      st.pr[q] = (double)s.x;
+      st.pi[q] = (double)s.y;
+    }
+  }
+  template <int E>
+  __device__ void emit(const State<E>& st, int g, int, C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    const double dtl = P->dt * P->bmob;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const KVec k = kfull(P, ck, tl + TPL * q);
+      const double den = 1.0 - dtl * kgk(P, g, k);
+      v[q] = mk<T>((T)((st.pr[q] - dtl * st.fr[q]) / den), (T)((st.pi[q] - dtl * st.fi[q]) / den));
+    }
+  }
+  template <int NW, class S> __device__ void reduce(const S&) const {}
+};
+
+}  // namespace mm

@@ -1,0 +1,46 @@
+// zfused.hip -- launchers of the fused z-pass kernels (zfused.h), dispatched
+// on the z length.
+#include "fft3d.h"
+#include "mm_common.h"
+#include "zfused.h"
+
+namespace mm {
+
+template <typename T, int L, class Op>
+static int go_zf(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int* nblk) {
+  using TL = TileZF<T, typename ZPlan<L, Op::LEAN4>::type, Op::MAXT>;
+  const int ncols = f.H * f.ny;
+  dim3 grid((ncols + TL::B - 1) / TL::B, ngroups);
+  hipLaunchKernelGGL((k_z_fused<T, L, Op>), grid, dim3(TL::THREADS), 0, s, op, (const C2<T>*)f.twNz, ncols);
+  if (nblk) *nblk = (int)grid.x;
+  return 0;
+}
+
+template <typename T, class Op>
+int launch_zfused(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int probe_kind, double bytes,
+                  int* nblk) {
+  if (f.probe) f.probe->begin(probe_kind, s);
+  switch (f.nz) {
+#define X(L_) case L_: go_zf<T, L_, Op>(f, op, ngroups, s, nblk); break;
+    X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+#undef X
+    default: return MM_ERR_ARG;
+  }
+  if (f.probe) f.probe->end(probe_kind, s, bytes);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+#define INST(T_, OP_) \
+  template int launch_zfused<T_, OP_<T_>>(const Fft3&, const OP_<T_>&, int, hipStream_t, int, double, int*);
+INST(double, OpGrad)
+INST(float, OpGrad)
+INST(double, OpElastic)
+INST(float, OpElastic)
+INST(double, OpCH)
+INST(float, OpCH)
+INST(double, OpAC)
+INST(float, OpAC)
+#undef INST
+
+}  // namespace mm
