@@ -50,6 +50,11 @@ def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind):
     shape = phi.shape[1:]
     sim = Simulation(params, precision)
     sim.upload(phi=phi, cons=cons, u=np.zeros((3, *shape)), gradu=np.zeros((9, *shape)))
+    var = (ctypes.c_int32 * 4)()
+    tms = (ctypes.c_float * 12)()
+    _lib.check(lib.mm_get_tuning(sim.ctx, var, tms), "mm_get_tuning")
+    tuning = {k: {"variant": var[i], "ms": [round(tms[3 * i + v], 4) for v in range(3)]}
+              for i, k in enumerate(["x_r2c", "x_c2r", "y", "z"])}
     sim.step(1 + warmup)  # elastep0 step + warm-up
     _lib.check(lib.mm_probe(sim.ctx, probe_kind), "mm_probe")
     torch.cuda.synchronize()
@@ -64,7 +69,7 @@ def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind):
     state = sim.download()
     sim.close()
     return dict(seconds=t1 - t0, rows=rows, probe_ms=ms.value, probe_launches=nl.value,
-                probe_bytes=by.value, state=state)
+                probe_bytes=by.value, state=state, tuning=tuning)
 
 
 def cpu_baseline(params, phi, cons, threads):
@@ -135,6 +140,7 @@ def main():
         "config": {"workload": f"{args.size}^3 c + 3 tetragonal variants, semi-implicit spectral step, FP64",
                    "grid": args.size, "variants": 3, "compositions": 1, "dt": 0.005, "seed": 7,
                    "elastic_iterations_per_step": n_avg, "parallelism": "single GPU"},
+        "fft_autotune": r64["tuning"],
     }
     if r64["probe_launches"]:
         avg_ms = r64["probe_ms"] / r64["probe_launches"]

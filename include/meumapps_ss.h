@@ -128,6 +128,31 @@ int mm_step(mm_ctx *ctx, int32_t nsteps, mm_diag *diag);
 int mm_get_step(const mm_ctx *ctx);
 int mm_set_step(mm_ctx *ctx, int32_t step);
 
+/* Graph mode (no reference counterpart): the regular step (step > elastep0)
+ * is recorded once into a HIP graph and replayed, with the elastic
+ * convergence control of ModElastic.f:435-481 running on the device over
+ * `iters` recorded iterations (0 keeps the current count, default 4); a step
+ * whose elastic loop needs more iterations finishes eagerly after the
+ * replay.  Results are identical to eager mode. */
+int mm_set_graph(mm_ctx *ctx, int32_t on, int32_t iters);
+
+/* Ensembles (no reference counterpart; the reference runs one simulation per
+ * MPI job, Multi_MEUMAPPS.f:548-1106).  n independent members, each with its
+ * own parameters and grid (params[0..n-1]) in one precision.  Fields go in
+ * and out through the member contexts (mm_batch_member + mm_upload /
+ * mm_download / mm_field / mm_set_step); those contexts belong to the batch
+ * and must not be destroyed separately.  mm_batch_step advances every member
+ * by nsteps; diag, when not NULL, holds nsteps * n rows, row k * n + i for
+ * member i at step k.  Steps where every member is past elastep0 run as one
+ * graph launch (one branch per member, graph_iters elastic iterations each as
+ * set on the member contexts before the first such step); other steps run the
+ * members one after another.  Each member's results equal its single run. */
+typedef struct mm_batch mm_batch;
+int mm_batch_create(int32_t n, const mm_params *params, int32_t precision, mm_batch **out);
+int mm_batch_member(mm_batch *batch, int32_t i, mm_ctx **ctx);
+int mm_batch_step(mm_batch *batch, int32_t nsteps, mm_diag *diag);
+void mm_batch_destroy(mm_batch *batch);
+
 /* Device-resident field access for in-process callers (torch, tests):
  * returns the device address of one SoA field and its element size (8 or 4).
  * which: 0 phi, 1 cons, 2 el_en, 3 f_diff (last step), 4 strain (6 fields:
@@ -157,6 +182,13 @@ int mm_field_to_host(mm_ctx *ctx, int32_t which, double *host);
 #define MM_PROBE_ZF_OTHER 9   /* fused z passes of gradients, CH, AC */
 int mm_probe(mm_ctx *ctx, int32_t kind);
 int mm_probe_read(mm_ctx *ctx, double *ms_total, int64_t *launches, double *bytes_total);
+
+/* FFT plan autotuner (no reference counterpart): at mm_create / mm_fft_create
+ * every radix plan of the x_r2c, x_c2r, y and z passes (default, lean,
+ * 4-point) is timed on the grid's own size and the fastest kept; set the
+ * environment variable MM_AUTOTUNE=0 to keep the defaults.  variant[4] gets
+ * the chosen plan per pass, ms[12] the measured kernel times (ms, 4 fields). */
+int mm_get_tuning(const mm_ctx *ctx, int32_t *variant, float *ms);
 
 /* ---- FFT interface: replaces f_trans / inv_trans over P3DFFT (ModFunc.f:15-40) ----
  * Spectrum layout (device): element (kx, ky, kz), kx in [0, Nx/2],

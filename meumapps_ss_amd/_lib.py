@@ -56,6 +56,12 @@ def load() -> ctypes.CDLL:
         "mm_errstr": (ctypes.c_char_p, [c_int32]),
         "mm_probe": (c_int32, [c_void_p, c_int32]),
         "mm_probe_read": (c_int32, [c_void_p, POINTER(c_double), POINTER(ctypes.c_int64), POINTER(c_double)]),
+        "mm_get_tuning": (c_int32, [c_void_p, POINTER(c_int32), POINTER(ctypes.c_float)]),
+        "mm_set_graph": (c_int32, [c_void_p, c_int32, c_int32]),
+        "mm_batch_create": (c_int32, [c_int32, c_void_p, c_int32, POINTER(c_void_p)]),
+        "mm_batch_member": (c_int32, [c_void_p, c_int32, POINTER(c_void_p)]),
+        "mm_batch_step": (c_int32, [c_void_p, c_int32, c_void_p]),
+        "mm_batch_destroy": (None, [c_void_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -74,5 +80,6 @@ EXPORTED = [
     "mm_create", "mm_destroy", "mm_upload", "mm_download", "mm_step", "mm_get_step",
     "mm_set_step", "mm_field", "mm_field_to_host", "mm_fft_create", "mm_fft_destroy",
     "mm_fft_forward", "mm_fft_inverse", "mm_fft_spectrum_elems", "mm_ran2", "mm_ic_uniform",
-    "mm_errstr", "mm_probe", "mm_probe_read",
+    "mm_errstr", "mm_probe", "mm_probe_read", "mm_get_tuning", "mm_set_graph",
+    "mm_batch_create", "mm_batch_member", "mm_batch_step", "mm_batch_destroy",
 ]

@@ -82,6 +82,27 @@ def workload(k: int, n: int = None) -> Workload:
     raise ValueError(k)
 
 
+def ensemble(sizes=(64, 96, 128), per_size: int = 16, seed0: int = 100, nsteps: int = 500,
+             draw_seed: int = 4):
+    """The ensemble workload: per_size members at each grid size of the
+    3-variant tetragonal model, member i with ran_2 seed seed0 + i, eigenstrain
+    magnitude m ~ U[0.005, 0.03] (diag(m, -m/2, -m/2) permuted per variant) and
+    kappa ~ U[0.5, 2.0], drawn from numpy's PCG64 with draw_seed."""
+    rng = np.random.default_rng(draw_seed)
+    out = []
+    i = 0
+    for n in sizes:
+        for _ in range(per_size):
+            m = float(rng.uniform(0.005, 0.03))
+            kappa = float(rng.uniform(0.5, 2.0))
+            seed = seed0 + i
+            inp = model_inputs(n, 3, (m, -0.5 * m, -0.5 * m), 0.005, kappa=kappa, nsteps=nsteps, seed=seed)
+            out.append(Workload(f"{n}^3 member {i}: eigenstrain {m:.4f}, kappa {kappa:.3f}", inp, seed, 0.01,
+                                (0.4,), nsteps))
+            i += 1
+    return out
+
+
 def initial_fields(w: Workload):
     """Uniform-noise initial fields from ran_2 (host side, through the C ABI)."""
     import ctypes

@@ -28,6 +28,14 @@ struct Probe {
 struct Fft3 {
   int nx = 0, ny = 0, nz = 0, H = 0, prec = 0;
   Probe* probe = nullptr;
+  // plan variant per pass (x_r2c, x_c2r, y, z): 0 default, 1 lean, 2 lean-4;
+  // chosen by autotune() from the kernel times in tune_ms
+  int var[4] = {0, 0, 0, 0};
+  float tune_ms[4][3] = {};
+  // when non-null every pass launched through this plan is gated on *gate
+  // (set while recording the graph-mode step)
+  const int* gate = nullptr;
+  int autotune(hipStream_t s);
   void* twLx = nullptr;  // exp(-2 pi i e/(Nx/2)), e < Nx/2
   void* twNx = nullptr;  // exp(-2 pi i e/Nx),     e < Nx
   void* twNy = nullptr;

@@ -1,6 +1,7 @@
 // fft3d.hip -- 3D real FFT plan, kernel dispatch by line length, and the
 // mm_fft_* entry points that stand in for f_trans / inv_trans (ModFunc.f:15-40).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -131,55 +132,71 @@ void Probe::destroy() {
 #define MM_NLENS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
 template <typename T>
-static Batch<T> make_batch(const void* const* in, void* const* out, int nf) {
+static Batch<T> make_batch(const void* const* in, void* const* out, int nf, const int* gate) {
   Batch<T> b;
   for (int i = 0; i < kMaxBatch; ++i) { b.in[i] = nullptr; b.out[i] = nullptr; }
   for (int i = 0; i < nf; ++i) { b.in[i] = in[i]; b.out[i] = out[i]; }
+  b.gate = gate;
   return b;
 }
 
-template <typename T, int L>
+// plan variant v of length L: 0 = Plan, 1 = PlanLean, 2 = PlanLean4
+template <int L, int V> struct PV { using type = Plan<L>; };
+template <int L> struct PV<L, 1> { using type = PlanLean<L>; };
+template <int L> struct PV<L, 2> { using type = PlanLean4<L>; };
+
+template <typename T, class P>
 static void go_x_r2c(const Batch<T>& b, int nf, const void* twL, const void* twN, int ny, int nz,
                      T scale, hipStream_t s) {
-  using TL = TileLM<T, L>;
+  using TL = TileLM<T, P>;
   dim3 grid((ny + TL::B - 1) / TL::B, nz, nf);
-  hipLaunchKernelGGL((k_x_r2c<T, L>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)twL,
+  hipLaunchKernelGGL((k_x_r2c<T, P>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)twL,
                      (const C2<T>*)twN, ny, scale);
 }
-template <typename T, int L>
+template <typename T, class P>
 static void go_x_c2r(const Batch<T>& b, int nf, const void* twL, const void* twN, int ny, int nz,
                      hipStream_t s) {
-  using TL = TileLM<T, L>;
+  using TL = TileLM<T, P>;
   dim3 grid((ny + TL::B - 1) / TL::B, nz, nf);
-  hipLaunchKernelGGL((k_x_c2r<T, L>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)twL,
+  hipLaunchKernelGGL((k_x_c2r<T, P>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)twL,
                      (const C2<T>*)twN, ny);
 }
-template <typename T, int L>
+template <typename T, class P>
 static void go_y(const Batch<T>& b, int nf, const void* tw, int nlines, int dir, hipStream_t s) {
-  using TL = TileLM<T, L>;
+  using TL = TileLM<T, P>;
   dim3 grid((nlines + TL::B - 1) / TL::B, 1, nf);
   if (dir < 0)
-    hipLaunchKernelGGL((k_y_pass<T, L, -1>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)tw, nlines);
+    hipLaunchKernelGGL((k_y_pass<T, P, -1>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)tw, nlines);
   else
-    hipLaunchKernelGGL((k_y_pass<T, L, 1>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)tw, nlines);
+    hipLaunchKernelGGL((k_y_pass<T, P, 1>), grid, dim3(TL::THREADS), 0, s, b, (const C2<T>*)tw, nlines);
 }
-template <typename T, int L>
+template <typename T, class P>
 static void go_z(const Batch<T>& b, int nf, const void* tw, int ncols, int dir, hipStream_t s) {
-  using TE = TileEM<T, L>;
+  using TE = TileEM<T, P>;
   dim3 grid((ncols + TE::B - 1) / TE::B, 1, nf);
   if (dir < 0)
-    hipLaunchKernelGGL((k_z_pass<T, L, -1>), grid, dim3(TE::THREADS), 0, s, b, (const C2<T>*)tw, ncols);
+    hipLaunchKernelGGL((k_z_pass<T, P, -1>), grid, dim3(TE::THREADS), 0, s, b, (const C2<T>*)tw, ncols);
   else
-    hipLaunchKernelGGL((k_z_pass<T, L, 1>), grid, dim3(TE::THREADS), 0, s, b, (const C2<T>*)tw, ncols);
+    hipLaunchKernelGGL((k_z_pass<T, P, 1>), grid, dim3(TE::THREADS), 0, s, b, (const C2<T>*)tw, ncols);
 }
+
+// variant dispatch: lengths below 64 have a single plan
+#define MM_VARIANTS(GO, L_, var, ...)                                              \
+  do {                                                                            \
+    if constexpr (L_ >= 64) {                                                     \
+      if ((var) == 1) { GO<T, typename PV<L_, 1>::type>(__VA_ARGS__); break; }    \
+      if ((var) == 2) { GO<T, typename PV<L_, 2>::type>(__VA_ARGS__); break; }    \
+    }                                                                             \
+    GO<T, typename PV<L_, 0>::type>(__VA_ARGS__);                                 \
+  } while (0)
 
 template <typename T>
 int Fft3::x_r2c(const T* const* in, void* const* out, int nf, hipStream_t s, T scale) const {
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
-  Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
+  Batch<T> b = make_batch<T>((const void* const*)in, out, nf, gate);
   if (probe) probe->begin(MM_PROBE_X_R2C, s);
   switch (nx / 2) {
-#define X(L_) case L_: go_x_r2c<T, L_>(b, nf, twLx, twNx, ny, nz, scale, s); break;
+#define X(L_) case L_: MM_VARIANTS(go_x_r2c, L_, var[0], b, nf, twLx, twNx, ny, nz, scale, s); break;
     MM_XLENS(X)
 #undef X
     default: return MM_ERR_ARG;
@@ -192,10 +209,10 @@ int Fft3::x_r2c(const T* const* in, void* const* out, int nf, hipStream_t s, T s
 template <typename T>
 int Fft3::x_c2r(void* const* in, T* const* out, int nf, hipStream_t s) const {
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
-  Batch<T> b = make_batch<T>((const void* const*)in, (void* const*)out, nf);
+  Batch<T> b = make_batch<T>((const void* const*)in, (void* const*)out, nf, gate);
   if (probe) probe->begin(MM_PROBE_X_C2R, s);
   switch (nx / 2) {
-#define X(L_) case L_: go_x_c2r<T, L_>(b, nf, twLx, twNx, ny, nz, s); break;
+#define X(L_) case L_: MM_VARIANTS(go_x_c2r, L_, var[1], b, nf, twLx, twNx, ny, nz, s); break;
     MM_XLENS(X)
 #undef X
     default: return MM_ERR_ARG;
@@ -208,11 +225,11 @@ int Fft3::x_c2r(void* const* in, T* const* out, int nf, hipStream_t s) const {
 template <typename T>
 int Fft3::y_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t s) const {
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
-  Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
+  Batch<T> b = make_batch<T>((const void* const*)in, out, nf, gate);
   int nlines = nz * H;
   if (probe) probe->begin(MM_PROBE_Y, s);
   switch (ny) {
-#define X(L_) case L_: go_y<T, L_>(b, nf, twNy, nlines, dir, s); break;
+#define X(L_) case L_: MM_VARIANTS(go_y, L_, var[2], b, nf, twNy, nlines, dir, s); break;
     MM_NLENS(X)
 #undef X
     default: return MM_ERR_ARG;
@@ -225,11 +242,11 @@ int Fft3::y_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t
 template <typename T>
 int Fft3::z_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t s) const {
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
-  Batch<T> b = make_batch<T>((const void* const*)in, out, nf);
+  Batch<T> b = make_batch<T>((const void* const*)in, out, nf, gate);
   int ncols = H * ny;
   if (probe) probe->begin(MM_PROBE_Z, s);
   switch (nz) {
-#define X(L_) case L_: go_z<T, L_>(b, nf, twNz, ncols, dir, s); break;
+#define X(L_) case L_: MM_VARIANTS(go_z, L_, var[3], b, nf, twNz, ncols, dir, s); break;
     MM_NLENS(X)
 #undef X
     default: return MM_ERR_ARG;
@@ -268,6 +285,69 @@ template int Fft3::y_pass<float>(void* const*, void* const*, int, int, hipStream
 template int Fft3::z_pass<double>(void* const*, void* const*, int, int, hipStream_t) const;
 template int Fft3::z_pass<float>(void* const*, void* const*, int, int, hipStream_t) const;
 
+// ---------------------------------------------------------------- autotuner
+// Times every plan variant of every pass on scratch fields of the plan's own
+// size (4 fields per launch, the typical batch) and keeps the fastest.
+template <typename T>
+static int autotune_t(Fft3& f, hipStream_t s) {
+  const int nf = 4;
+  void* re = nullptr;
+  void* sp = nullptr;
+  if (hipMalloc(&re, nf * f.nreal() * sizeof(T)) != hipSuccess) return MM_ERR_NOMEM;
+  if (hipMalloc(&sp, nf * f.ncplx() * 2 * sizeof(T)) != hipSuccess) { (void)hipFree(re); return MM_ERR_NOMEM; }
+  MM_HIP(hipMemsetAsync(re, 0, nf * f.nreal() * sizeof(T), s));
+  MM_HIP(hipMemsetAsync(sp, 0, nf * f.ncplx() * 2 * sizeof(T), s));
+  const T* rin[4];
+  T* rout[4];
+  void* sps[4];
+  for (int i = 0; i < nf; ++i) {
+    rin[i] = (const T*)re + i * f.nreal();
+    rout[i] = (T*)re + i * f.nreal();
+    sps[i] = (C2<T>*)sp + i * f.ncplx();
+  }
+  hipEvent_t a, b;
+  MM_HIP(hipEventCreate(&a));
+  MM_HIP(hipEventCreate(&b));
+  Probe* keep = f.probe;
+  f.probe = nullptr;
+  const int lens[4] = {f.nx / 2, f.nx / 2, f.ny, f.nz};
+  int rc = 0;
+  for (int k = 0; k < 4 && !rc; ++k) {
+    if (lens[k] < 64) { f.var[k] = 0; continue; }
+    float best = 1e30f;
+    int bestv = 0;
+    for (int v = 0; v < 3 && !rc; ++v) {
+      f.var[k] = v;
+      float tmin = 1e30f;
+      for (int rep = 0; rep < 3 && !rc; ++rep) {
+        MM_HIP(hipEventRecord(a, s));
+        if (k == 0) rc = f.x_r2c<T>(rin, sps, nf, s, (T)1);
+        else if (k == 1) rc = f.x_c2r<T>(sps, rout, nf, s);
+        else if (k == 2) rc = f.y_pass<T>(sps, sps, nf, -1, s);
+        else rc = f.z_pass<T>(sps, sps, nf, -1, s);
+        MM_HIP(hipEventRecord(b, s));
+        MM_HIP(hipEventSynchronize(b));
+        float t = 0.f;
+        MM_HIP(hipEventElapsedTime(&t, a, b));
+        if (rep > 0 && t < tmin) tmin = t;
+      }
+      f.tune_ms[k][v] = tmin;
+      if (tmin < best) { best = tmin; bestv = v; }
+    }
+    f.var[k] = bestv;
+  }
+  f.probe = keep;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(re);
+  (void)hipFree(sp);
+  return rc;
+}
+
+int Fft3::autotune(hipStream_t s) {
+  return prec == MM_FP64 ? autotune_t<double>(*this, s) : autotune_t<float>(*this, s);
+}
+
 }  // namespace mm
 
 // ---------------------------------------------------------------- C ABI
@@ -282,6 +362,8 @@ extern "C" int mm_fft_create(int32_t nx, int32_t ny, int32_t nz, int32_t precisi
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MM_ERR_NODEV;
   mm_fft* h = new mm_fft();
   int rc = h->f.init(nx, ny, nz, precision);
+  const char* at = std::getenv("MM_AUTOTUNE");
+  if (!rc && !(at && at[0] == '0')) rc = h->f.autotune(nullptr);
   if (rc) {
     h->f.destroy();
     delete h;

@@ -20,10 +20,15 @@
 
 namespace mm {
 
+// Field pointers of one batched launch.  gate (may be null) lets a kernel
+// that was recorded into a HIP graph skip its work when the device-side
+// control of the step says so (the whole workgroup returns before any barrier).
 template <typename T> struct Batch {
   const void* in[kMaxBatch];
   void* out[kMaxBatch];
+  const int* gate;
 };
+__device__ __forceinline__ bool gated_off(const int* g) { return g != nullptr && *g == 0; }
 
 __host__ __device__ constexpr int pow2floor(int x) {
   int p = 1;
@@ -36,15 +41,15 @@ constexpr int kLdsBudget = 65536;
 constexpr int kMaxThreads = 256;
 
 // lines per workgroup, line-major LDS (x and y passes)
-template <typename T, int L> struct TileLM {
-  static constexpr int TPL = Plan<L>::TPL;
+template <typename T, class P> struct TileLM {
+  static constexpr int L = P::L, TPL = P::TPL;
   static constexpr int B = pow2floor(cmin(kMaxThreads / TPL, kLdsBudget / (line_pitch<L>() * (int)sizeof(C2<T>))));
   static constexpr int THREADS = B * TPL;
   static constexpr int LDS = B * line_pitch<L>();
 };
 // columns per workgroup, element-major LDS (z pass)
-template <typename T, int L> struct TileEM {
-  static constexpr int TPL = Plan<L>::TPL;
+template <typename T, class P> struct TileEM {
+  static constexpr int L = P::L, TPL = P::TPL;
   static constexpr int B = pow2floor(cmin(kMaxThreads / TPL, kLdsBudget / (L * (int)sizeof(C2<T>))));
   static constexpr int THREADS = B * TPL;
   static constexpr int LDS = B * L;
@@ -52,12 +57,12 @@ template <typename T, int L> struct TileEM {
 
 // ------------------------------------------------------------------ x r2c
 // grid (ceil(ny/B), nz, nfields); L = Nx/2
-template <typename T, int L>
-__global__ void __launch_bounds__((TileLM<T, L>::THREADS))
+template <typename T, class P>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS))
 k_x_r2c(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T scale) {
-  using P = Plan<L>;
-  constexpr int TPL = P::TPL, E = P::E, B = TileLM<T, L>::B, LP = line_pitch<L>();
-  __shared__ C2<T> lds[TileLM<T, L>::LDS];
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  if (gated_off(bt.gate)) return;
   const int f = blockIdx.z, z = blockIdx.y, y0 = blockIdx.x * B, t = threadIdx.x;
   const T* __restrict__ in = (const T*)bt.in[f];
   C2<T>* __restrict__ out = (C2<T>*)bt.out[f];
@@ -102,12 +107,12 @@ k_x_r2c(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ tw
 }
 
 // ------------------------------------------------------------------ x c2r
-template <typename T, int L>
-__global__ void __launch_bounds__((TileLM<T, L>::THREADS))
+template <typename T, class P>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS))
 k_x_c2r(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny) {
-  using P = Plan<L>;
-  constexpr int TPL = P::TPL, E = P::E, B = TileLM<T, L>::B, LP = line_pitch<L>();
-  __shared__ C2<T> lds[TileLM<T, L>::LDS];
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  if (gated_off(bt.gate)) return;
   const int f = blockIdx.z, z = blockIdx.y, y0 = blockIdx.x * B, t = threadIdx.x;
   const C2<T>* __restrict__ in = (const C2<T>*)bt.in[f];
   T* __restrict__ out = (T*)bt.out[f];
@@ -153,12 +158,12 @@ k_x_c2r(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ tw
 
 // -------------------------------------------------- y pass (contiguous lines)
 // grid (ceil(nlines/B), 1, nfields); line l = kz*(Nx/2+1) + kx
-template <typename T, int L, int DIR>
-__global__ void __launch_bounds__((TileLM<T, L>::THREADS))
+template <typename T, class P, int DIR>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS))
 k_y_pass(Batch<T> bt, const C2<T>* __restrict__ twL, int nlines) {
-  using P = Plan<L>;
-  constexpr int TPL = P::TPL, E = P::E, B = TileLM<T, L>::B, LP = line_pitch<L>();
-  __shared__ C2<T> lds[TileLM<T, L>::LDS];
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  if (gated_off(bt.gate)) return;
   const int f = blockIdx.z, t = threadIdx.x;
   const int b = t / TPL, tl = t % TPL, line = blockIdx.x * B + b;
   // in and out may be the same buffer (in-place pass): no __restrict__
@@ -182,12 +187,12 @@ k_y_pass(Batch<T> bt, const C2<T>* __restrict__ twL, int nlines) {
 
 // ----------------------------------------------------- z pass (strided lines)
 // grid (ceil(ncols/B), 1, nfields); column c = kx*Ny + ky, stride ncols
-template <typename T, int L, int DIR>
-__global__ void __launch_bounds__((TileEM<T, L>::THREADS))
+template <typename T, class P, int DIR>
+__global__ void __launch_bounds__((TileEM<T, P>::THREADS))
 k_z_pass(Batch<T> bt, const C2<T>* __restrict__ twL, int ncols) {
-  using P = Plan<L>;
-  constexpr int TPL = P::TPL, E = P::E, B = TileEM<T, L>::B;
-  __shared__ C2<T> lds[TileEM<T, L>::LDS];
+  constexpr int TPL = P::TPL, E = P::E, B = TileEM<T, P>::B;
+  __shared__ C2<T> lds[TileEM<T, P>::LDS];
+  if (gated_off(bt.gate)) return;
   const int f = blockIdx.z, t = threadIdx.x;
   const int b = t % B, tl = t / B, col = blockIdx.x * B + b;
   const C2<T>* in = (const C2<T>*)bt.in[f] + col;

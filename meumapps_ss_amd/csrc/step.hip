@@ -21,7 +21,9 @@ int launch_zfused(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int p
 // ============================================================== kernels
 
 __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ part, int nblk, int K,
-                                                     FinalizeOps ops, double* __restrict__ out) {
+                                                     FinalizeOps ops, double* __restrict__ out,
+                                                     const int* __restrict__ gate) {
+  if (gated_off(gate)) return;
   __shared__ double sh[kWaves];
   for (int k = 0; k < K; ++k) {
     const int op = ops.op[k];
@@ -33,6 +35,35 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ 
     double r = op == 0 ? block_reduce<0>(acc, sh) : (op == 1 ? block_reduce<1>(acc, sh) : block_reduce<2>(acc, sh));
     if (threadIdx.x == 0) out[k] = r;
   }
+}
+
+// result slots in dres
+enum { R_FINT = 0, R_FCH, R_FW, R_EL, R_ERR0 /* 6 */, R_CLAMP = 10 /* E+4 */, R_N = 24 };
+
+// graph-mode helpers: fill, and the one-thread elastic control
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill(T* __restrict__ p, size_t n, T v, const int* __restrict__ gate) {
+  if (gated_off(gate)) return;
+  for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) p[i] = v;
+}
+__global__ void k_ctl_start(Ctl* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl_start(*c);
+}
+__global__ void k_ctl_iter(Ctl* c, const double* __restrict__ err) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl_after_iter(*c, err);
+}
+
+// Stage layout after a replay (doubles): [0, 12) the Ctl block, [16, 16 + R_N)
+// the result slots, [kStageErr] the error flag.
+constexpr int kStage = 64, kStageRes = 16, kStageErr = 48;
+static_assert(sizeof(Ctl) == 12 * sizeof(double), "Ctl block is 12 doubles");
+static_assert(kStageRes + R_N <= kStageErr, "stage layout");
+__global__ void k_gather(const Ctl* __restrict__ ctl, const double* __restrict__ dres, const int* __restrict__ derr,
+                         double* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t < 12) out[t] = reinterpret_cast<const double*>(ctl)[t];
+  if (t < R_N) out[kStageRes + t] = dres[t];
+  if (t == 0) out[kStageErr] = (double)*derr;
 }
 
 #define GRID_STRIDE(i, n) \
@@ -299,7 +330,10 @@ __device__ __forceinline__ void compliance_mix(const DevParams* __restrict__ P, 
 template <typename T, int V, int NPH, bool HOMOG>
 __global__ void __launch_bounds__(kBlock)
 k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ strain,
-        double cfac, T* __restrict__ xpol, int* __restrict__ err, size_t n3) {
+        double cfac_arg, const double* __restrict__ cfac_dev, T* __restrict__ xpol, int* __restrict__ err,
+        size_t n3, const int* __restrict__ gate) {
+  if (gated_off(gate)) return;
+  const double cfac = cfac_dev ? *cfac_dev : cfac_arg;
   bool bad = false;
   GRID_STRIDE(pt, n3) {
     double Pv[V], ps = 0.0;
@@ -358,7 +392,8 @@ template <typename T, int V, int NPH, bool HOMOG>
 __global__ void __launch_bounds__(kBlock)
 k_elener(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ strain,
          T* __restrict__ el_en, T* __restrict__ elgrad, T* __restrict__ fdiff,
-         double* __restrict__ part, int* __restrict__ err, size_t n3) {
+         double* __restrict__ part, int* __restrict__ err, size_t n3, const int* __restrict__ gate) {
+  if (gated_off(gate)) return;
   __shared__ double sh[kWaves];
   double s_el = 0.0;
   bool bad = false;
@@ -460,7 +495,8 @@ k_elener(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __
 template <typename T, int V, int E>
 __global__ void __launch_bounds__(kBlock)
 k_clamp(const DevParams* __restrict__ P, T* __restrict__ phi, const T* __restrict__ cons,
-        double* __restrict__ part, size_t n3) {
+        double* __restrict__ part, size_t n3, const int* __restrict__ gate) {
+  if (gated_off(gate)) return;
   __shared__ double sh[kWaves];
   double cs[E], cmin = HUGE_VAL, cmax = -HUGE_VAL, pmax = -HUGE_VAL, sf = 0.0;
 #pragma unroll
@@ -671,9 +707,11 @@ k_ks_deriv(const DevParams* __restrict__ P, const C2<T>* __restrict__ in, C2<T>*
 template <typename T> struct Kern {
   void (*chem)(dim3, hipStream_t, const DevParams*, const T*, const T*, const T*, const T*, T*, T*, double*, int*, size_t) = nullptr;
   void (*ppoly)(dim3, hipStream_t, const T*, T*, size_t) = nullptr;
-  void (*polar)(dim3, hipStream_t, const DevParams*, const T*, const T*, double, T*, int*, size_t) = nullptr;
-  void (*elener)(dim3, hipStream_t, const DevParams*, const T*, const T*, T*, T*, T*, double*, int*, size_t) = nullptr;
-  void (*clamp)(dim3, hipStream_t, const DevParams*, T*, const T*, double*, size_t) = nullptr;
+  void (*polar)(dim3, hipStream_t, const DevParams*, const T*, const T*, double, const double*, T*, int*, size_t,
+                const int*) = nullptr;
+  void (*elener)(dim3, hipStream_t, const DevParams*, const T*, const T*, T*, T*, T*, double*, int*, size_t,
+                 const int*) = nullptr;
+  void (*clamp)(dim3, hipStream_t, const DevParams*, T*, const T*, double*, size_t, const int*) = nullptr;
   void (*zeroth)(dim3, hipStream_t, const DevParams*, const C2<T>*, C2<T>*, C2<T>*, size_t) = nullptr;
 };
 
@@ -686,17 +724,18 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
     hipLaunchKernelGGL((k_ppoly<T, V>), g, dim3(kBlock), 0, s, a, b, n);
   }
   template <bool H>
-  static void polar(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, double cf, T* x,
-                    int* err, size_t n) {
-    hipLaunchKernelGGL((k_polar<T, V, NPH, H>), g, dim3(kBlock), 0, s, P, a, b, cf, x, err, n);
+  static void polar(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, double cf,
+                    const double* cfd, T* x, int* err, size_t n, const int* gate) {
+    hipLaunchKernelGGL((k_polar<T, V, NPH, H>), g, dim3(kBlock), 0, s, P, a, b, cf, cfd, x, err, n, gate);
   }
   template <bool H>
   static void elener(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, T* c, T* d, T* e,
-                     double* p, int* err, size_t n) {
-    hipLaunchKernelGGL((k_elener<T, V, NPH, H>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, p, err, n);
+                     double* p, int* err, size_t n, const int* gate) {
+    hipLaunchKernelGGL((k_elener<T, V, NPH, H>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, p, err, n, gate);
   }
-  static void clamp(dim3 g, hipStream_t s, const DevParams* P, T* a, const T* b, double* p, size_t n) {
-    hipLaunchKernelGGL((k_clamp<T, V, E>), g, dim3(kBlock), 0, s, P, a, b, p, n);
+  static void clamp(dim3 g, hipStream_t s, const DevParams* P, T* a, const T* b, double* p, size_t n,
+                    const int* gate) {
+    hipLaunchKernelGGL((k_clamp<T, V, E>), g, dim3(kBlock), 0, s, P, a, b, p, n, gate);
   }
   static void zeroth(dim3 g, hipStream_t s, const DevParams* P, const C2<T>* a, C2<T>* b, C2<T>* c, size_t n) {
     hipLaunchKernelGGL((k_ks_zeroth<T, V>), g, dim3(kBlock), 0, s, P, a, b, c, n);
@@ -753,8 +792,18 @@ struct mm_ctx {
   Kern<float> k32;
   Probe probe;
   bool zfused = true;  // z passes with the k-space operations fused in (MM_ZFUSED=0: separate kernels)
+  bool autotune = true;  // time the FFT plan variants at creation (MM_AUTOTUNE=0: defaults)
   int last_niter = 0;
   double last_cfac = 1.0;
+  // graph mode: the regular step (step > elastep0) recorded once and replayed
+  Ctl* ctl = nullptr;       // device control of the elastic iteration
+  double* dstage = nullptr;  // kStage doubles: Ctl, the result sums and the error flag after a replay
+  double* hstage = nullptr;  // pinned host copy
+  const int* gate = nullptr;  // gate applied to launches while recording
+  bool graph = false;       // mm_set_graph
+  int graph_iters = 4;      // elastic iterations recorded per step
+  hipGraph_t g = nullptr;
+  hipGraphExec_t gx = nullptr;
 };
 
 namespace {
@@ -762,8 +811,6 @@ namespace {
 constexpr int kMaxPartSlots = 8;
 constexpr int kMaxBlocks = 2048;      // grid of the pointwise and k-space kernels
 constexpr int kMaxPartBlocks = 65536; // largest grid that writes partial sums
-// result slots in dres
-enum { R_FINT = 0, R_FCH, R_FW, R_EL, R_ERR0 /* 6 */, R_CLAMP = 10 /* E+4 */, R_N = 24 };
 
 int blocks_for(size_t n) { return (int)std::min<size_t>((n + kBlock - 1) / kBlock, (size_t)kMaxBlocks); }
 
@@ -837,9 +884,14 @@ int finalize(mm_ctx* c, int nblk, int K, std::initializer_list<int> ops, int slo
   int i = 0;
   for (int o : ops) fo.op[i++] = o;
   for (; i < 8; ++i) fo.op[i] = 0;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->s, c->part, nblk, K, fo, c->dres + slot0);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->s, c->part, nblk, K, fo, c->dres + slot0, c->gate);
   MM_HIP(hipGetLastError());
   return 0;
+}
+
+void set_gate(mm_ctx* c, const int* g) {
+  c->gate = g;
+  c->fft.gate = g;
 }
 
 template <typename T> const Kern<T>& kern(const mm_ctx* c);
@@ -885,110 +937,96 @@ template <typename T> int inv_yx(mm_ctx* c, C2<T>* const* sp, T* const* out, int
   return 0;
 }
 
-// ------------------------------------------------------------------ eldis
-template <typename T> int eldis(mm_ctx* c) {
+// ------------------------------------------------------------ elasticity
+// zeroth-order displacement at step == elastep0 (ModElastic.f:105-236)
+template <typename T> int elastic_zeroth(mm_ctx* c) {
   const size_t n3 = c->n3, nc = c->nc;
   const Kern<T>& K = kern<T>(c);
-  const dim3 gp(c->nblk_pw), gk(c->nblk_k);
-  T* phi = (T*)c->phi;
-  T* strain = (T*)c->strain;
-  T* xpol = (T*)c->xpol;
   C2<T>* work = (C2<T>*)c->swork;
-  C2<T>* uhat = (C2<T>*)c->suhat;
   C2<T>* wsp[6];
   T* wre[6];
-  for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = strain + (size_t)m * n3; }
-  if (c->step == c->prm.elastep0) {
-    K.ppoly(gp, c->s, phi, xpol, n3);
-    MM_HIP(hipGetLastError());
-    C2<T>* phat = work + (size_t)6 * nc;
-    MM_TRY(fwd<T>(c, xpol, n3, phat, c->V));
-    K.zeroth(gk, c->s, c->dp, phat, uhat, work, nc);
-    MM_HIP(hipGetLastError());
-    MM_TRY(inv<T>(c, wsp, wre, 6));
-  }
-  double newfac = 0.5, cfac = 1.0, err_old[6] = {0, 0, 0, 0, 0, 0};
-  int niter = 0, total = 0;
-  bool converged = false;
-  for (;;) {
-    niter += 1;
-    total += 1;
-    c->probe.begin(MM_PROBE_POLAR, c->s);
-    K.polar(gp, c->s, c->dp, phi, strain, cfac, xpol, c->derr, n3);
-    c->probe.end(MM_PROBE_POLAR, c->s, (double)n3 * sizeof(T) * (c->V + 12));
-    MM_HIP(hipGetLastError());
-    if (c->zfused) {
-      const T* xin[6];
-      for (int m = 0; m < 6; ++m) xin[m] = xpol + (size_t)m * n3;
-      MM_TRY(fwd_xy<T>(c, xin, wsp, 6));
-      OpElastic<T> op{c->dp, work, uhat, c->part, nc};
-      int nblk = 0;
-      MM_TRY(launch_zfused<T>(c->fft, op, 1, c->s, MM_PROBE_ZF_ELASTIC, (double)nc * 2 * sizeof(T) * 18, &nblk));
-      MM_TRY(finalize(c, nblk, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
-    } else {
-      MM_TRY(fwd<T>(c, xpol, n3, work, 6));
-      c->probe.begin(MM_PROBE_KS_ELASTIC, c->s);
-      hipLaunchKernelGGL(k_ks_elastic<T>, gk, dim3(kBlock), 0, c->s, c->dp, work, uhat, c->part, nc);
-      c->probe.end(MM_PROBE_KS_ELASTIC, c->s, (double)nc * 2 * sizeof(T) * 18);
-      MM_HIP(hipGetLastError());
-      MM_TRY(finalize(c, c->nblk_k, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
-    }
-    MM_HIP(hipMemcpyAsync(c->hres + R_ERR0, c->dres + R_ERR0, 6 * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    MM_HIP(hipEventRecord(c->ev, c->s));
-    if (c->zfused) {
-      MM_TRY(inv_yx<T>(c, wsp, wre, 6));
-    } else {
-      MM_TRY(inv<T>(c, wsp, wre, 6));
-    }
-    MM_HIP(hipEventSynchronize(c->ev));
-    const double* err = c->hres + R_ERR0;
-    // convergence / backtracking control, ModElastic.f:435-481
-    if (err[0] / err[1] < 1.e-7 && err[2] / err[3] < 1.e-7 && err[4] / err[5] < 1.e-7) converged = true;
-    if (niter > 1 && !converged) {
-      if (err[0] / err[1] > err_old[0] / err_old[1] || err[2] / err[3] > err_old[2] / err_old[3] ||
-          err[4] / err[5] > err_old[4] / err_old[5]) {
-        cfac = cfac * newfac;
-        newfac = newfac * 0.5;
-        if (newfac < 1.e-7) return MM_ERR_NEWFAC;
-        niter = 0;
-        converged = false;
-        continue;
-      }
-    }
-    for (int m = 0; m < 6; ++m) err_old[m] = err[m];
-    if (niter == 10) converged = true;
-    if (!converged || niter < 4) continue;
-    if (cfac < 1.0 - 1.e-6) {
-      cfac = cfac + 0.1;
-      if (cfac > 1.0) cfac = 1.0;
-      niter = 0;
-      converged = false;
-      continue;
-    }
-    break;
-  }
-  c->last_niter = total;
-  c->last_cfac = cfac;
-  c->probe.begin(MM_PROBE_ELENER, c->s);
-  K.elener(gp, c->s, c->dp, phi, strain, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->derr, n3);
-  c->probe.end(MM_PROBE_ELENER, c->s, (double)n3 * sizeof(T) * (4 * c->V + 7));
+  for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
+  K.ppoly(dim3(c->nblk_pw), c->s, (T*)c->phi, (T*)c->xpol, n3);
   MM_HIP(hipGetLastError());
-  MM_TRY(finalize(c, c->nblk_pw, 1, {0}, R_EL));
+  C2<T>* phat = work + (size_t)6 * nc;
+  MM_TRY(fwd<T>(c, (T*)c->xpol, n3, phat, c->V));
+  K.zeroth(dim3(c->nblk_k), c->s, c->dp, phat, (C2<T>*)c->suhat, work, nc);
+  MM_HIP(hipGetLastError());
+  MM_TRY(inv<T>(c, wsp, wre, 6));
   return 0;
 }
 
+// One fixed-point iteration (ModElastic.f:241-433) up to the six convergence
+// sums in dres[R_ERR0..]; the strain inverse transforms follow separately.
+// cfac comes from the argument (eager) or from the device control (graph).
+template <typename T> int elastic_iter(mm_ctx* c, double cfac, const double* cfac_dev) {
+  const size_t n3 = c->n3, nc = c->nc;
+  const Kern<T>& K = kern<T>(c);
+  T* xpol = (T*)c->xpol;
+  C2<T>* work = (C2<T>*)c->swork;
+  C2<T>* uhat = (C2<T>*)c->suhat;
+  c->probe.begin(MM_PROBE_POLAR, c->s);
+  K.polar(dim3(c->nblk_pw), c->s, c->dp, (T*)c->phi, (T*)c->strain, cfac, cfac_dev, xpol, c->derr, n3, c->gate);
+  c->probe.end(MM_PROBE_POLAR, c->s, (double)n3 * sizeof(T) * (c->V + 12));
+  MM_HIP(hipGetLastError());
+  if (c->zfused) {
+    const T* xin[6];
+    C2<T>* wsp[6];
+    for (int m = 0; m < 6; ++m) { xin[m] = xpol + (size_t)m * n3; wsp[m] = work + (size_t)m * nc; }
+    MM_TRY(fwd_xy<T>(c, xin, wsp, 6));
+    OpElastic<T> op{c->dp, work, uhat, c->part, nc};
+    op.gate = c->gate;
+    int nblk = 0;
+    MM_TRY(launch_zfused<T>(c->fft, op, 1, c->s, MM_PROBE_ZF_ELASTIC, (double)nc * 2 * sizeof(T) * 18, &nblk));
+    MM_TRY(finalize(c, nblk, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
+  } else {
+    MM_TRY(fwd<T>(c, xpol, n3, work, 6));
+    c->probe.begin(MM_PROBE_KS_ELASTIC, c->s);
+    hipLaunchKernelGGL(k_ks_elastic<T>, dim3(c->nblk_k), dim3(kBlock), 0, c->s, c->dp, work, uhat, c->part, nc);
+    c->probe.end(MM_PROBE_KS_ELASTIC, c->s, (double)nc * 2 * sizeof(T) * 18);
+    MM_HIP(hipGetLastError());
+    MM_TRY(finalize(c, c->nblk_k, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
+  }
+  return 0;
+}
+
+template <typename T> int elastic_strain_inverse(mm_ctx* c) {
+  const size_t n3 = c->n3, nc = c->nc;
+  C2<T>* work = (C2<T>*)c->swork;
+  C2<T>* wsp[6];
+  T* wre[6];
+  for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
+  if (c->zfused) return inv_yx<T>(c, wsp, wre, 6);
+  return inv<T>(c, wsp, wre, 6);
+}
+
+// eager iterations with host control, starting from the control state h
+template <typename T> int elastic_loop_eager(mm_ctx* c, Ctl& h) {
+  while (h.active) {
+    MM_TRY(elastic_iter<T>(c, h.cfac, nullptr));
+    MM_HIP(hipMemcpyAsync(c->hres + R_ERR0, c->dres + R_ERR0, 6 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    MM_HIP(hipEventRecord(c->ev, c->s));
+    MM_TRY(elastic_strain_inverse<T>(c));
+    MM_HIP(hipEventSynchronize(c->ev));
+    ctl_after_iter(h, c->hres + R_ERR0);
+  }
+  c->last_niter = h.total;
+  c->last_cfac = h.cfac;
+  return h.err;
+}
+
 // -------------------------------------------------------------- one step
-template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
+// head: phi_hat, gradient block, chemistry (Multi_MEUMAPPS.f:575-845)
+template <typename T> int step_head(mm_ctx* c) {
   const size_t n3 = c->n3, nc = c->nc;
   const int V = c->V, E = c->E;
   const Kern<T>& K = kern<T>(c);
-  const dim3 gp(c->nblk_pw);
   T* phi = (T*)c->phi;
   C2<T>* sphi = (C2<T>*)c->sphi;
   C2<T>* work = (C2<T>*)c->swork;
-  if (c->step <= c->prm.noise_step) return MM_ERR_ARG;  // Langevin noise is host-side, not supported here
-  MM_HIP(hipMemsetAsync(c->derr, 0, sizeof(int), c->s));
-  // 1. phi_hat (Multi_MEUMAPPS.f:577-578); kept for the Allen-Cahn update
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(kBlock), 0, c->s, c->derr, (size_t)1, 0, c->gate);
+  MM_HIP(hipGetLastError());
+  // 1. phi_hat (:577-578); kept for the Allen-Cahn update
   if (c->zfused) {
     const T* pin[MM_MAX_VAR];
     C2<T>* pout[MM_MAX_VAR];
@@ -1003,6 +1041,7 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
     const int g = std::min(G, V - v0);
     if (c->zfused) {
       OpGrad<T> op{c->dp, sphi, work, nc, v0};
+      op.gate = c->gate;
       MM_TRY(launch_zfused<T>(c->fft, op, g, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 6 * g, nullptr));
     } else {
       hipLaunchKernelGGL(k_ks_grad<T>, dim3(c->nblk_k, g), dim3(kBlock), 0, c->s, c->dp, sphi, work, v0, nc);
@@ -1024,22 +1063,37 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
   }
   // 3. chemistry (:634-845)
   c->probe.begin(MM_PROBE_CHEM, c->s);
-  K.chem(gp, c->s, c->dp, phi, (T*)c->gphi, (T*)c->kdel, (T*)c->cons, (T*)c->fdiff, (T*)c->flux, c->part,
-         c->derr, n3);
+  K.chem(dim3(c->nblk_pw), c->s, c->dp, phi, (T*)c->gphi, (T*)c->kdel, (T*)c->cons, (T*)c->fdiff, (T*)c->flux,
+         c->part, c->derr, n3);
   c->probe.end(MM_PROBE_CHEM, c->s, (double)n3 * sizeof(T) * (6 * V + 4 * E));
   MM_HIP(hipGetLastError());
   MM_TRY(finalize(c, c->nblk_pw, 3, {0, 0, 0}, R_FINT));
-  // 4. elasticity (:851-880)
-  c->last_niter = 0;
-  c->last_cfac = 1.0;
-  if (c->step >= c->prm.elastep0) {
-    MM_TRY(eldis<T>(c));
+  return 0;
+}
+
+// tail: elener (when the elastic solve ran), Cahn-Hilliard, Allen-Cahn, clamp
+template <typename T> int step_tail(mm_ctx* c, bool elastic) {
+  const size_t n3 = c->n3, nc = c->nc;
+  const int V = c->V, E = c->E;
+  const Kern<T>& K = kern<T>(c);
+  T* phi = (T*)c->phi;
+  C2<T>* sphi = (C2<T>*)c->sphi;
+  C2<T>* work = (C2<T>*)c->swork;
+  const dim3 gp(c->nblk_pw);
+  if (elastic) {
+    c->probe.begin(MM_PROBE_ELENER, c->s);
+    K.elener(gp, c->s, c->dp, phi, (T*)c->strain, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->derr, n3,
+             c->gate);
+    c->probe.end(MM_PROBE_ELENER, c->s, (double)n3 * sizeof(T) * (4 * c->V + 7));
+    MM_HIP(hipGetLastError());
+    MM_TRY(finalize(c, c->nblk_pw, 1, {0}, R_EL));
   } else {
-    MM_HIP(hipMemsetAsync(c->el_en, 0, n3 * sizeof(T), c->s));
-    MM_HIP(hipMemsetAsync(c->elgrad, 0, (size_t)V * n3 * sizeof(T), c->s));
-    MM_HIP(hipMemsetAsync(c->dres + R_EL, 0, sizeof(double), c->s));
+    hipLaunchKernelGGL(k_fill<T>, gp, dim3(kBlock), 0, c->s, (T*)c->el_en, n3, (T)0, c->gate);
+    hipLaunchKernelGGL(k_fill<T>, gp, dim3(kBlock), 0, c->s, (T*)c->elgrad, (size_t)V * n3, (T)0, c->gate);
+    hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(kBlock), 0, c->s, c->dres + R_EL, (size_t)1, 0.0, c->gate);
+    MM_HIP(hipGetLastError());
   }
-  // 5. Cahn-Hilliard (:910-949): work[4e] = c_hat, work[4e+1..3] = F_hat
+  // Cahn-Hilliard (:910-949): work[4e] = c_hat, work[4e+1..3] = F_hat
   {
     const T scale = (T)(1.0 / (double)(float)n3);
     for (int e = 0; e < E; ++e) {
@@ -1055,6 +1109,7 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
     }
     if (c->zfused) {
       OpCH<T> opch{c->dp, work, nc};
+      opch.gate = c->gate;
       MM_TRY(launch_zfused<T>(c->fft, opch, E, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 5 * E, nullptr));
     } else {
       hipLaunchKernelGGL(k_ks_ch<T>, dim3(c->nblk_k, E), dim3(kBlock), 0, c->s, c->dp, work, nc);
@@ -1069,7 +1124,7 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
       MM_TRY(inv<T>(c, sp, rp, E));
     }
   }
-  // 6. Allen-Cahn (:955-1001)
+  // Allen-Cahn (:955-1001)
   {
     C2<T>* sp[MM_MAX_VAR];
     T* rp[MM_MAX_VAR];
@@ -1079,6 +1134,7 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
       for (int v = 0; v < V; ++v) { fin[v] = (T*)c->fdiff + (size_t)v * n3; fout[v] = work + (size_t)v * nc; }
       MM_TRY(fwd_xy<T>(c, fin, fout, V));
       OpAC<T> opac{c->dp, work, sphi, nc};
+      opac.gate = c->gate;
       MM_TRY(launch_zfused<T>(c->fft, opac, V, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 3 * V, nullptr));
       for (int v = 0; v < V; ++v) { sp[v] = work + (size_t)v * nc; rp[v] = phi + (size_t)v * n3; }
       MM_TRY(inv_yx<T>(c, sp, rp, V));
@@ -1090,25 +1146,25 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
       MM_TRY(inv<T>(c, sp, rp, V));
     }
   }
-  // 7. clamp and check-loop sums (:1008-1065)
-  K.clamp(gp, c->s, c->dp, phi, (T*)c->cons, c->part, n3);
+  // clamp and check-loop sums (:1008-1065)
+  K.clamp(gp, c->s, c->dp, phi, (T*)c->cons, c->part, n3, c->gate);
   MM_HIP(hipGetLastError());
-  {
-    FinalizeOps fo;
-    for (int e = 0; e < E; ++e) fo.op[e] = 0;
-    fo.op[E] = 1; fo.op[E + 1] = 2; fo.op[E + 2] = 2; fo.op[E + 3] = 0;
-    for (int i = E + 4; i < 8; ++i) fo.op[i] = 0;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->s, c->part, c->nblk_pw, E + 4, fo, c->dres + R_CLAMP);
-    MM_HIP(hipGetLastError());
-  }
-  MM_HIP(hipMemcpyAsync(c->hres, c->dres, R_N * sizeof(double), hipMemcpyDeviceToHost, c->s));
-  int herr = 0;
-  MM_HIP(hipMemcpyAsync(c->hres + R_N, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->s));
-  MM_HIP(hipStreamSynchronize(c->s));
+  FinalizeOps fo;
+  for (int e = 0; e < E; ++e) fo.op[e] = 0;
+  fo.op[E] = 1; fo.op[E + 1] = 2; fo.op[E + 2] = 2; fo.op[E + 3] = 0;
+  for (int i = E + 4; i < 8; ++i) fo.op[i] = 0;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->s, c->part, c->nblk_pw, E + 4, fo, c->dres + R_CLAMP,
+                     c->gate);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+// fill the fort.98/99 row from the result slots r, advance `step`
+int step_account(mm_ctx* c, const double* r, int herr, mm_diag* d) {
+  const int E = c->E;
+  const size_t n3 = c->n3;
   MM_TRY(c->probe.collect());
-  std::memcpy(&herr, c->hres + R_N, sizeof(int));
   if (herr) return herr;
-  const double* r = c->hres;
   if (d) {
     std::memset(d, 0, sizeof(*d));
     d->step = c->step;
@@ -1123,8 +1179,7 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
     d->glob_max = c->glob_max;
     d->phi_max = c->phi_max;
     d->solid_tot = c->solid_frac / (double)(float)n3;
-    for (int e = 0; e < E; ++e)
-      d->caviter[e] = c->hres[R_CLAMP + e] * (1.0 / (double)(float)n3) + c->hp.cbar[e];
+    for (int e = 0; e < E; ++e) d->caviter[e] = r[R_CLAMP + e] * (1.0 / (double)(float)n3) + c->hp.cbar[e];
   }
   // lagged check-loop values for the next row (con_min starts at 1, con_max and max_phi at 0)
   c->glob_min = std::min(1.0, r[R_CLAMP + E]);
@@ -1135,12 +1190,117 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
   return 0;
 }
 
+// copy the step's results to the host, then account them
+int step_collect(mm_ctx* c, mm_diag* d) {
+  MM_HIP(hipMemcpyAsync(c->hres, c->dres, R_N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  MM_HIP(hipMemcpyAsync(c->hres + R_N, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  MM_HIP(hipStreamSynchronize(c->s));
+  int herr = 0;
+  std::memcpy(&herr, c->hres + R_N, sizeof(int));
+  return step_account(c, c->hres, herr, d);
+}
+
+// eager step: host-driven elastic control
+template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
+  if (c->step <= c->prm.noise_step) return MM_ERR_ARG;  // Langevin noise is host-side, not supported here
+  set_gate(c, nullptr);
+  MM_TRY(step_head<T>(c));
+  const bool elastic = c->step >= c->prm.elastep0;
+  c->last_niter = 0;
+  c->last_cfac = 1.0;
+  if (elastic) {
+    if (c->step == c->prm.elastep0) MM_TRY(elastic_zeroth<T>(c));
+    Ctl h;
+    ctl_start(h);
+    MM_TRY(elastic_loop_eager<T>(c, h));
+  }
+  MM_TRY(step_tail<T>(c, elastic));
+  return step_collect(c, d);
+}
+
+// Record the regular step (step > elastep0) on c->s while it is capturing:
+// head, graph_iters gated iterations with device control, gated tail.
+template <typename T> int record_step(mm_ctx* c, double* stage) {
+  set_gate(c, nullptr);
+  MM_TRY(step_head<T>(c));
+  hipLaunchKernelGGL(k_ctl_start, dim3(1), dim3(1), 0, c->s, c->ctl);
+  MM_HIP(hipGetLastError());
+  for (int i = 0; i < c->graph_iters; ++i) {
+    set_gate(c, &c->ctl->active);
+    MM_TRY(elastic_iter<T>(c, 0.0, &c->ctl->cfac));
+    set_gate(c, nullptr);
+    hipLaunchKernelGGL(k_ctl_iter, dim3(1), dim3(1), 0, c->s, c->ctl, (const double*)(c->dres + R_ERR0));
+    MM_HIP(hipGetLastError());
+    set_gate(c, &c->ctl->ran);
+    MM_TRY(elastic_strain_inverse<T>(c));
+  }
+  set_gate(c, &c->ctl->done);
+  MM_TRY(step_tail<T>(c, true));
+  set_gate(c, nullptr);
+  hipLaunchKernelGGL(k_gather, dim3(1), dim3(64), 0, c->s, (const Ctl*)c->ctl, (const double*)c->dres,
+                     (const int*)c->derr, stage);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+// After a replay, from the staged control block and sums st: if the recorded
+// iterations did not finish the elastic loop (backtracking, slow
+// convergence), continue eagerly from the device state and run the tail.
+template <typename T> int finish_replay(mm_ctx* c, const double* st, mm_diag* d) {
+  Ctl h;
+  std::memcpy(&h, st, sizeof(Ctl));
+  if (!h.done) {
+    set_gate(c, nullptr);
+    MM_TRY(elastic_loop_eager<T>(c, h));
+    MM_TRY(step_tail<T>(c, true));
+    return step_collect(c, d);
+  }
+  if (h.err) return h.err;
+  c->last_niter = h.total;
+  c->last_cfac = h.cfac;
+  return step_account(c, st + kStageRes, (int)st[kStageErr], d);
+}
+
+int graph_build(mm_ctx* c, hipStream_t cap);
+int graph_ready(mm_ctx* c);
+
+template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
+  const bool regular = c->step > c->prm.elastep0 && c->step > c->prm.noise_step;
+  if (!(c->graph && c->zfused && regular)) return run_step_eager<T>(c, d);
+  if (!c->gx) MM_TRY(graph_build(c, c->s));
+  MM_HIP(hipGraphLaunch(c->gx, c->s));
+  MM_HIP(hipMemcpyAsync(c->hstage, c->dstage, kStage * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  MM_HIP(hipStreamSynchronize(c->s));
+  return finish_replay<T>(c, c->hstage, d);
+}
+
+int graph_build(mm_ctx* c, hipStream_t cap) {
+  const int kind = c->probe.kind;
+  c->probe.kind = -1;  // no event records inside the graph
+  MM_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  int rc = c->prec == MM_FP64 ? record_step<double>(c, c->dstage) : record_step<float>(c, c->dstage);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(cap, &g);
+  c->probe.kind = kind;
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  c->g = g;
+  MM_HIP(hipGraphInstantiate(&c->gx, c->g, nullptr, nullptr, 0));
+  return 0;
+}
+
 void free_all(mm_ctx* c) {
+  if (c->gx) (void)hipGraphExecDestroy(c->gx);
+  if (c->g) (void)hipGraphDestroy(c->g);
+  c->gx = nullptr;
+  c->g = nullptr;
   void* bufs[] = {c->phi, c->cons, c->gphi, c->kdel, c->fdiff, c->flux, c->strain, c->xpol, c->el_en,
-                  c->elgrad, c->sphi, c->suhat, c->swork, c->part, c->dres, c->derr, c->dp};
+                  c->elgrad, c->sphi, c->suhat, c->swork, c->part, c->dres, c->derr, c->dp, c->ctl,
+                  c->dstage};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
+  if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->ev) (void)hipEventDestroy(c->ev);
   c->probe.destroy();
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -1303,6 +1463,11 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
     const char* zf = std::getenv("MM_ZFUSED");
     c->zfused = !(zf && zf[0] == '0');
   }
+  c->autotune = true;
+  {
+    const char* at = std::getenv("MM_AUTOTUNE");
+    if (at && at[0] == '0') c->autotune = false;
+  }
   if (rc) { free_all(c); delete c; return rc; }
   c->n3 = c->fft.nreal();
   c->nc = c->fft.ncplx();
@@ -1331,10 +1496,15 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   rc = rc ? rc : dalloc((void**)&c->dres, 64 * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->derr, 16);
   rc = rc ? rc : dalloc((void**)&c->dp, sizeof(DevParams));
+  rc = rc ? rc : dalloc((void**)&c->ctl, sizeof(Ctl));
+  rc = rc ? rc : dalloc((void**)&c->dstage, kStage * sizeof(double));
+  if (!rc && hipHostMalloc((void**)&c->hstage, kStage * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    rc = MM_ERR_NOMEM;
   if (!rc && hipHostMalloc((void**)&c->hres, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = MM_ERR_NOMEM;
   if (!rc && hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) rc = MM_ERR_HIP;
   if (!rc && hipEventCreateWithFlags(&c->ev, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
   if (!rc && hipMemcpy(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess) rc = MM_ERR_HIP;
+  if (!rc && c->autotune) rc = c->fft.autotune(c->s);
   if (!rc) {
     // zero state so that a step without upload is well defined
     void* z[] = {c->phi, c->cons, c->strain, c->el_en, c->elgrad, c->suhat};
@@ -1438,5 +1608,149 @@ extern "C" int mm_probe_read(mm_ctx* c, double* ms, int64_t* launches, double* b
   const int kind = c->probe.kind;
   c->probe.reset();
   c->probe.kind = kind;
+  return 0;
+}
+
+extern "C" int mm_get_tuning(const mm_ctx* c, int32_t* variant, float* ms) {
+  if (!c) return MM_ERR_ARG;
+  for (int k = 0; k < 4; ++k) {
+    if (variant) variant[k] = c->fft.var[k];
+    for (int v = 0; v < 3; ++v)
+      if (ms) ms[k * 3 + v] = c->fft.tune_ms[k][v];
+  }
+  return 0;
+}
+
+extern "C" int mm_set_graph(mm_ctx* c, int32_t on, int32_t iters) {
+  if (!c || iters < 0 || iters > 64) return MM_ERR_ARG;
+  if (c->gx) { (void)hipGraphExecDestroy(c->gx); c->gx = nullptr; }
+  if (c->g) { (void)hipGraphDestroy(c->g); c->g = nullptr; }
+  c->graph = on != 0;
+  if (iters > 0) c->graph_iters = iters;
+  return 0;
+}
+
+// ---------------------------------------------------------------- ensembles
+// Independent members (own parameters, grid and fields) advanced together:
+// the regular step of every member is recorded into one graph with one
+// branch per member, launched once per step.
+struct mm_batch {
+  std::vector<mm_ctx*> m;
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  double* dstage = nullptr;
+  double* hstage = nullptr;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t gx = nullptr;
+  int prec = MM_FP64;
+};
+
+static void batch_free(mm_batch* b) {
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  if (b->gx) (void)hipGraphExecDestroy(b->gx);
+  if (b->g) (void)hipGraphDestroy(b->g);
+  for (mm_ctx* c : b->m) mm_destroy(c);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->dstage) (void)hipFree(b->dstage);
+  if (b->hstage) (void)hipHostFree(b->hstage);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  delete b;
+}
+
+extern "C" int mm_batch_create(int32_t n, const mm_params* params, int32_t precision, mm_batch** out) {
+  if (!out || !params || n < 1 || n > 4096) return MM_ERR_ARG;
+  *out = nullptr;
+  mm_batch* b = new mm_batch;
+  b->prec = precision;
+  int rc = 0;
+  for (int i = 0; i < n && !rc; ++i) {
+    mm_ctx* c = nullptr;
+    rc = mm_create(params + i, precision, &c);
+    if (!rc) b->m.push_back(c);
+  }
+  if (!rc && hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess) rc = MM_ERR_HIP;
+  if (!rc && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
+  for (int i = 0; i < n && !rc; ++i) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { rc = MM_ERR_HIP; break; }
+    b->join.push_back(e);
+  }
+  if (!rc && hipMalloc((void**)&b->dstage, (size_t)n * kStage * sizeof(double)) != hipSuccess) rc = MM_ERR_NOMEM;
+  if (!rc && hipHostMalloc((void**)&b->hstage, (size_t)n * kStage * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    rc = MM_ERR_NOMEM;
+  if (rc) { batch_free(b); return rc; }
+  *out = b;
+  return 0;
+}
+
+extern "C" void mm_batch_destroy(mm_batch* b) {
+  if (b) batch_free(b);
+}
+
+extern "C" int mm_batch_member(mm_batch* b, int32_t i, mm_ctx** ctx) {
+  if (!b || !ctx || i < 0 || i >= (int)b->m.size()) return MM_ERR_ARG;
+  *ctx = b->m[i];
+  return 0;
+}
+
+static int batch_build(mm_batch* b) {
+  MM_HIP(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+  int rc = hipEventRecord(b->fork, b->s) == hipSuccess ? 0 : MM_ERR_HIP;
+  for (size_t i = 0; i < b->m.size() && !rc; ++i) {
+    mm_ctx* c = b->m[i];
+    if (hipStreamWaitEvent(c->s, b->fork, 0) != hipSuccess) { rc = MM_ERR_HIP; break; }
+    const int kind = c->probe.kind;
+    c->probe.kind = -1;
+    double* st = b->dstage + i * kStage;
+    rc = c->prec == MM_FP64 ? record_step<double>(c, st) : record_step<float>(c, st);
+    c->probe.kind = kind;
+    if (!rc && hipEventRecord(b->join[i], c->s) != hipSuccess) rc = MM_ERR_HIP;
+    if (!rc && hipStreamWaitEvent(b->s, b->join[i], 0) != hipSuccess) rc = MM_ERR_HIP;
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->s, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  b->g = g;
+  MM_HIP(hipGraphInstantiate(&b->gx, b->g, nullptr, nullptr, 0));
+  return 0;
+}
+
+template <typename T> static int batch_step_one(mm_batch* b, mm_diag* d) {
+  const size_t n = b->m.size();
+  bool regular = true;
+  for (mm_ctx* c : b->m)
+    regular = regular && c->zfused && c->step > c->prm.elastep0 && c->step > c->prm.noise_step;
+  if (!regular) {
+    for (size_t i = 0; i < n; ++i) {
+      mm_ctx* c = b->m[i];
+      MM_TRY(c->prec == MM_FP64 ? run_step_eager<double>(c, d ? d + i : nullptr)
+                                : run_step_eager<float>(c, d ? d + i : nullptr));
+    }
+    return 0;
+  }
+  if (!b->gx) MM_TRY(batch_build(b));
+  MM_HIP(hipGraphLaunch(b->gx, b->s));
+  MM_HIP(hipMemcpyAsync(b->hstage, b->dstage, n * kStage * sizeof(double), hipMemcpyDeviceToHost, b->s));
+  MM_HIP(hipStreamSynchronize(b->s));
+  int first = 0;
+  for (size_t i = 0; i < n; ++i) {
+    mm_ctx* c = b->m[i];
+    const int rc = c->prec == MM_FP64 ? finish_replay<double>(c, b->hstage + i * kStage, d ? d + i : nullptr)
+                                      : finish_replay<float>(c, b->hstage + i * kStage, d ? d + i : nullptr);
+    if (rc && !first) first = rc;
+  }
+  return first;
+}
+
+extern "C" int mm_batch_step(mm_batch* b, int32_t nsteps, mm_diag* diag) {
+  if (!b || nsteps < 0) return MM_ERR_ARG;
+  const size_t n = b->m.size();
+  for (int k = 0; k < nsteps; ++k) {
+    mm_diag* d = diag ? diag + (size_t)k * n : nullptr;
+    MM_TRY(b->prec == MM_FP64 ? batch_step_one<double>(b, d) : batch_step_one<float>(b, d));
+  }
   return 0;
 }

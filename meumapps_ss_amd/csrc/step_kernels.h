@@ -42,6 +42,71 @@ struct DevParams {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
+// State of the elastic fixed-point iteration of one step (ModElastic.f:100-103,
+// 435-481).  The same update runs on the host (eager steps) and in a one-thread
+// kernel (graph-recorded steps), so both paths take identical decisions.
+struct Ctl {
+  double cfac, newfac;
+  double err_old[6];
+  int niter, total, converged;
+  int active;  // another iteration follows
+  int ran;     // the iteration just finished ran (gates its inverse transforms)
+  int done;    // the loop has ended (gates the rest of the step)
+  int err;     // MM_ERR_NEWFAC when backtracking gave up
+  int pad;
+};
+
+__host__ __device__ inline void ctl_start(Ctl& c) {
+  c.cfac = 1.0;
+  c.newfac = 0.5;
+  for (int m = 0; m < 6; ++m) c.err_old[m] = 0.0;
+  c.niter = 0;
+  c.total = 0;
+  c.converged = 0;
+  c.active = 1;
+  c.ran = 0;
+  c.done = 0;
+  c.err = 0;
+  c.pad = 0;
+}
+
+// After one iteration with the six sums err = {sum (u1-u1_old)^2, sum u1_old^2, ...}.
+__host__ __device__ inline void ctl_after_iter(Ctl& c, const double* err) {
+  c.ran = c.active;
+  if (!c.active) return;
+  c.niter += 1;
+  c.total += 1;
+  if (err[0] / err[1] < 1.e-7 && err[2] / err[3] < 1.e-7 && err[4] / err[5] < 1.e-7) c.converged = 1;
+  if (c.niter > 1 && !c.converged) {
+    if (err[0] / err[1] > c.err_old[0] / c.err_old[1] || err[2] / err[3] > c.err_old[2] / c.err_old[3] ||
+        err[4] / err[5] > c.err_old[4] / c.err_old[5]) {
+      c.cfac = c.cfac * c.newfac;
+      c.newfac = c.newfac * 0.5;
+      if (c.newfac < 1.e-7) {
+        c.err = MM_ERR_NEWFAC;
+        c.active = 0;
+        c.done = 1;
+        return;
+      }
+      c.niter = 0;
+      c.converged = 0;
+      return;
+    }
+  }
+  for (int m = 0; m < 6; ++m) c.err_old[m] = err[m];
+  if (c.niter == 10) c.converged = 1;
+  if (!c.converged || c.niter < 4) return;
+  if (c.cfac < 1.0 - 1.e-6) {
+    c.cfac = c.cfac + 0.1;
+    if (c.cfac > 1.0) c.cfac = 1.0;
+    c.niter = 0;
+    c.converged = 0;
+    return;
+  }
+  c.active = 0;
+  c.done = 1;
+}
+
 template <typename T> __device__ __forceinline__ double ld(const T* p, size_t i) { return (double)p[i]; }
 template <typename T> __device__ __forceinline__ void st(T* p, size_t i, double v) { p[i] = (T)v; }
 

@@ -66,6 +66,7 @@ __global__ void __launch_bounds__((TileZF<T, typename ZPlan<L, Op::LEAN4>::type,
 k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
   using P = typename ZPlan<L, Op::LEAN4>::type;
   using TL = TileZF<T, P, Op::MAXT>;
+  if (gated_off(op.gate)) return;
   constexpr int TPL = P::TPL, E = P::E, B = TL::B;
   __shared__ C2<T> lds[TL::LDS];
   const int t = threadIdx.x, b = t % B, tl = t / B, col = blockIdx.x * B + b, g = blockIdx.y;
@@ -74,12 +75,23 @@ k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
   const ColK ck = colk(op.P, colc);
   IdxElemMajor idx{b, B};
   typename Op::template State<E> st;
+  // the column of input f+1 is loaded while input f is transformed
+  C2<T> vn[E];
+  {
+    const C2<T>* in = op.input(g, 0) + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) vn[q] = valid ? in[(size_t)(tl + TPL * q) * ncols] : mk<T>(0, 0);
+  }
 #pragma unroll 1
   for (int f = 0; f < Op::NIN; ++f) {
     C2<T> v[E];
-    const C2<T>* in = op.input(g, f) + col;
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = valid ? in[(size_t)(tl + TPL * q) * ncols] : mk<T>(0, 0);
+    for (int q = 0; q < E; ++q) v[q] = vn[q];
+    if (f + 1 < Op::NIN) {
+      const C2<T>* in = op.input(g, f + 1) + col;
+#pragma unroll
+      for (int q = 0; q < E; ++q) vn[q] = valid ? in[(size_t)(tl + TPL * q) * ncols] : mk<T>(0, 0);
+    }
     if (f > 0) __syncthreads();
     const C2<T>* tw = twL;
     int tlx = tl;
@@ -118,6 +130,7 @@ template <typename T> struct OpGrad {
   C2<T>* work;
   size_t nc;
   int v0;
+  const int* gate = nullptr;
   template <int E> struct State { C2<T> s[E]; };
   __device__ const C2<T>* input(int g, int) const { return sphi + (size_t)(v0 + g) * nc; }
   __device__ C2<T>* output(int g, int o) const { return work + (size_t)(4 * g + o) * nc; }
@@ -173,6 +186,7 @@ template <typename T> struct OpElastic {
   C2<T>* uhat;
   double* part;
   size_t nc;
+  const int* gate = nullptr;
   template <int E> struct State {
     double rr[3][E], ri[3][E];  // R, then u
     double acc[6];
@@ -294,6 +308,7 @@ template <typename T> struct OpCH {
   const DevParams* P;
   C2<T>* work;
   size_t nc;
+  const int* gate = nullptr;
   template <int E> struct State { double cr[E], ci[E], tr[E], ti[E]; };
   __device__ const C2<T>* input(int g, int f) const { return work + (size_t)(4 * g + f) * nc; }
   __device__ C2<T>* output(int g, int) const { return work + (size_t)(4 * g) * nc; }
@@ -349,6 +364,7 @@ template <typename T> struct OpAC {
   C2<T>* work;
   const C2<T>* sphi;
   size_t nc;
+  const int* gate = nullptr;
   template <int E> struct State { double fr[E], fi[E], pr[E], pi[E]; };
   __device__ const C2<T>* input(int g, int) const { return work + (size_t)g * nc; }
   __device__ C2<T>* output(int g, int) const { return work + (size_t)g * nc; }

@@ -35,19 +35,27 @@ def aos_to_soa(a: np.ndarray, ncomp: int, shape) -> np.ndarray:
 
 
 class Simulation:
-    def __init__(self, params: MMParams, precision: int = MM_FP64):
+    def __init__(self, params: MMParams, precision: int = MM_FP64, _ctx: Optional[ctypes.c_void_p] = None):
         self.lib = _lib.load()
         self.p = params
         self.precision = precision
-        self.ctx = ctypes.c_void_p()
-        _lib.check(self.lib.mm_create(ctypes.byref(params), precision, ctypes.byref(self.ctx)), "mm_create")
+        self.owned = _ctx is None
+        if self.owned:
+            self.ctx = ctypes.c_void_p()
+            _lib.check(self.lib.mm_create(ctypes.byref(params), precision, ctypes.byref(self.ctx)), "mm_create")
+        else:
+            self.ctx = _ctx
         self.shape = (params.nz, params.ny, params.nx)
         self.V, self.E = params.vartot, params.numel
 
     def close(self):
-        if self.ctx:
+        if self.ctx and self.owned:
             self.lib.mm_destroy(self.ctx)
-            self.ctx = ctypes.c_void_p()
+        self.ctx = ctypes.c_void_p()
+
+    def set_graph(self, on: bool = True, iters: int = 0):
+        """Replay the regular step as one recorded graph (results identical)."""
+        _lib.check(self.lib.mm_set_graph(self.ctx, int(on), iters), "mm_set_graph")
 
     def __enter__(self):
         return self
@@ -111,6 +119,51 @@ class Simulation:
         rows = (MMDiag * n)()
         _lib.check(self.lib.mm_step(self.ctx, n, rows), "mm_step")
         return [r.as_dict(self.E) for r in rows]
+
+
+class Ensemble:
+    """Independent members advanced together, one graph launch per regular
+    step (``mm_batch_*``).  ``members[i]`` is a ``Simulation`` view of member
+    i for upload / download / field access; the batch owns the contexts."""
+
+    def __init__(self, params: List[MMParams], precision: int = MM_FP64):
+        self.lib = _lib.load()
+        n = len(params)
+        arr = (MMParams * n)(*params)
+        self.b = ctypes.c_void_p()
+        _lib.check(self.lib.mm_batch_create(n, ctypes.cast(arr, ctypes.c_void_p), precision,
+                                            ctypes.byref(self.b)), "mm_batch_create")
+        self.members = []
+        for i, p in enumerate(params):
+            c = ctypes.c_void_p()
+            _lib.check(self.lib.mm_batch_member(self.b, i, ctypes.byref(c)), "mm_batch_member")
+            self.members.append(Simulation(p, precision, _ctx=c))
+
+    def step(self, n: int = 1) -> List[List[dict]]:
+        """Advance every member n steps; returns rows[k][i] for member i."""
+        m = len(self.members)
+        rows = (MMDiag * (n * m))()
+        _lib.check(self.lib.mm_batch_step(self.b, n, ctypes.cast(rows, ctypes.c_void_p)), "mm_batch_step")
+        return [[rows[k * m + i].as_dict(self.members[i].E) for i in range(m)] for k in range(n)]
+
+    def close(self):
+        if self.b:
+            for s in self.members:
+                s.ctx = ctypes.c_void_p()
+            self.lib.mm_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def fort98_line(d: dict) -> str:

@@ -1,0 +1,113 @@
+"""Graph-mode step and ensembles against the eager step.
+
+The graph replays the regular step with the elastic convergence control on
+the device; every replay is checked against an eager context fed the same
+inputs, with host downloads (eager work and synchronisation) between the
+replays.  Both paths launch the same kernels on the same data in the same
+order, so the bar is bit equality; the eager FP64 path itself is held to the
+oracle by the parity tests.
+"""
+import numpy as np
+import pytest
+
+from meumapps_ss_amd.configs import model_inputs
+from meumapps_ss_amd.params import MM_FP64, MM_MIXED, setup
+from meumapps_ss_amd.sim import Ensemble, Simulation
+from parity_cases import make_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _zeros(shape):
+    return dict(u=np.zeros((3, *shape)), gradu=np.zeros((9, *shape)))
+
+
+def _same(a, b, what):
+    for k in ("phi", "cons", "u", "gradu", "el_en"):
+        assert np.array_equal(a[k], b[k]), f"{what}: {k} differs, max {np.abs(a[k] - b[k]).max():.3e}"
+
+
+def _rows_same(r, q, what):
+    for k in ("step", "n_elastic", "fint", "fch", "fw", "elen", "etot", "glob_min", "glob_max", "phi_max",
+              "solid_tot", "cfac"):
+        assert r[k] == q[k], f"{what}: row {k} {r[k]} != {q[k]}"
+    assert list(r["caviter"]) == list(q["caviter"]), what
+
+
+@pytest.mark.parametrize("name,prec,iters", [("v3_tetra", MM_FP64, 4), ("v3_inhomog", MM_FP64, 4),
+                                             ("v2_twophase", MM_FP64, 4), ("v1_e2", MM_FP64, 4),
+                                             ("v3_tetra", MM_MIXED, 4), ("v3_inhomog", MM_FP64, 1),
+                                             ("v2_nonsq", MM_MIXED, 2)])
+def test_graph_replays_match_eager(gpu_lib, name, prec, iters):
+    p, phi, cons = make_case(name, seed=2)
+    shape = phi.shape[1:]
+    with Simulation(p, prec) as e, Simulation(p, prec) as g:
+        e.upload(phi=phi, cons=cons, **_zeros(shape))
+        g.upload(phi=phi, cons=cons, **_zeros(shape))
+        g.set_graph(True, iters)
+        for k in range(6):  # step 1 (elastep0) is eager in both; 2..6 replay the graph
+            re_, rg = e.step(1)[0], g.step(1)[0]
+            _rows_same(rg, re_, f"{name} step {k + 1}")
+            _same(g.download(), e.download(), f"{name} step {k + 1}")
+
+
+def test_graph_off_again_and_restart(gpu_lib):
+    """Switching graph mode off and on, and a restart (set_step) between
+    replays, keep the graph path equal to the eager one."""
+    p, phi, cons = make_case("v3_tetra", seed=3)
+    shape = phi.shape[1:]
+    with Simulation(p, MM_FP64) as e, Simulation(p, MM_FP64) as g:
+        for s in (e, g):
+            s.upload(phi=phi, cons=cons, **_zeros(shape))
+        g.set_graph(True)
+        e.step(3); g.step(3)
+        g.set_graph(False)
+        e.step(1); g.step(1)
+        g.set_graph(True, 3)
+        e.step(2); g.step(2)
+        _same(g.download(), e.download(), "toggle")
+        st = e.download()
+        for s in (e, g):
+            s.upload(phi=st["phi"], cons=st["cons"], u=st["u"], gradu=st["gradu"])
+            s.set_step(p.elastep0 + 1)
+        e.step(2); g.step(2)
+        _same(g.download(), e.download(), "restart")
+
+
+def _member_params(sizes, seed0=200):
+    ws = []
+    rng = np.random.default_rng(5)
+    for i, n in enumerate(sizes):
+        m = float(rng.uniform(0.005, 0.03))
+        kappa = float(rng.uniform(0.5, 2.0))
+        inp = model_inputs(n, 3, (m, -0.5 * m, -0.5 * m), 0.005, kappa=kappa, seed=seed0 + i)
+        ws.append(inp)
+    return ws
+
+
+@pytest.mark.parametrize("prec", [MM_MIXED, MM_FP64])
+def test_ensemble_members_equal_single_runs(gpu_lib, prec):
+    sizes = (16, 24, 32, 16, 24)
+    inps = _member_params(sizes)
+    params = [setup(i) for i in inps]
+    rng = np.random.default_rng(9)
+    fields = []
+    for n in sizes:
+        fields.append((rng.uniform(-0.01, 0.01, (3, n, n, n)) + 0.3 * (rng.random((3, n, n, n)) > 0.9),
+                       0.4 + rng.uniform(-0.01, 0.01, (1, n, n, n))))
+    nsteps = 5
+    with Ensemble(params, prec) as ens:
+        for m, (phi, cons) in zip(ens.members, fields):
+            m.upload(phi=phi, cons=cons, **_zeros(phi.shape[1:]))
+        rows = ens.step(2)
+        for k in range(nsteps - 2):  # downloads between replays
+            rows += ens.step(1)
+            _ = ens.members[k % len(sizes)].download()
+        got = [m.download() for m in ens.members]
+    for i, (p, (phi, cons)) in enumerate(zip(params, fields)):
+        with Simulation(p, prec) as s:
+            s.upload(phi=phi, cons=cons, **_zeros(phi.shape[1:]))
+            ref = s.step(nsteps)
+            _same(got[i], s.download(), f"member {i}")
+        for k in range(nsteps):
+            _rows_same(rows[k][i], ref[k], f"member {i} step {k + 1}")

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from meumapps_ss_amd.configs import JMOL, model_inputs, workload
+from meumapps_ss_amd.configs import JMOL, ensemble, model_inputs, workload
 from meumapps_ss_amd.params import (F32_1055E5, SetupError, eltensor, read_input_txt, setup,
                                     write_input_txt)
 
@@ -82,3 +82,19 @@ def test_workloads_build():
         w = workload(k, n=16)
         p = w.params()
         assert p.nx == 16 and p.vartot >= 1
+
+
+def test_ensemble_workload_definition():
+    ws = ensemble()
+    assert len(ws) == 48
+    assert [w.inputs.nx for w in ws] == [64] * 16 + [96] * 16 + [128] * 16
+    assert [w.seed for w in ws] == list(range(100, 148))
+    ms, ks = [], []
+    for w in ws:
+        t = w.inputs.phases[1].tstr
+        assert 0.005 <= t[0] <= 0.03 and t[1] == t[2] == -0.5 * t[0]
+        ks.append(5 * w.inputs.phases[1].eintf[0])
+        ms.append(t[0])
+        p = w.params()
+        assert p.nx == w.inputs.nx and p.vartot == 3
+    assert 0.5 <= min(ks) and max(ks) <= 2.0 and len(set(ks)) == 48 and len(set(ms)) == 48
