@@ -39,7 +39,7 @@ def step_bytes_per_cell(V, E, n, w):
     return w * (2 * nfft + npw)
 
 
-def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind):
+def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind, graph=False):
     import numpy as np
     import torch
 
@@ -50,6 +50,8 @@ def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind):
     shape = phi.shape[1:]
     sim = Simulation(params, precision)
     sim.upload(phi=phi, cons=cons, u=np.zeros((3, *shape)), gradu=np.zeros((9, *shape)))
+    if graph:
+        sim.set_graph(True)
     var = (ctypes.c_int32 * 4)()
     tms = (ctypes.c_float * 12)()
     _lib.check(lib.mm_get_tuning(sim.ctx, var, tms), "mm_get_tuning")
@@ -70,6 +72,55 @@ def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind):
     sim.close()
     return dict(seconds=t1 - t0, rows=rows, probe_ms=ms.value, probe_launches=nl.value,
                 probe_bytes=by.value, state=state, tuning=tuning)
+
+
+def run_ensemble(steps, warmup, graph=True, per_size=16):
+    """The 48-member ensemble (16 each at 64^3, 96^3, 128^3, MIXED): one graph
+    launch per step for all members, or (graph=False) the members stepped one
+    after another through the eager path."""
+    import numpy as np
+    import torch
+
+    from meumapps_ss_amd.configs import ensemble, initial_fields
+    from meumapps_ss_amd.params import MM_MIXED
+    from meumapps_ss_amd.sim import Ensemble, Simulation
+
+    ws = ensemble(per_size=per_size)
+    params = [w.params() for w in ws]
+    cells = sum(p.nx * p.ny * p.nz for p in params)
+    t_create = time.perf_counter()
+    if graph:
+        ens = Ensemble(params, MM_MIXED)
+        members = ens.members
+    else:
+        ens = None
+        members = [Simulation(p, MM_MIXED) for p in params]
+    t_create = time.perf_counter() - t_create
+    for m, w in zip(members, ws):
+        phi, cons = initial_fields(w)
+        shape = phi.shape[1:]
+        m.upload(phi=phi, cons=cons, u=np.zeros((3, *shape)), gradu=np.zeros((9, *shape)))
+
+    def adv(k):
+        if graph:
+            ens.step(k)
+        else:
+            for _ in range(k):
+                for m in members:
+                    m.step(1)
+
+    adv(1 + warmup)  # elastep0 step (eager) and warm-up; the first regular step records the graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    adv(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if graph:
+        ens.close()
+    else:
+        for m in members:
+            m.close()
+    return dict(seconds=t1 - t0, cells=cells, members=len(params), create_s=t_create)
 
 
 def cpu_baseline(params, phi, cons, threads):
@@ -96,9 +147,15 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--ensemble-steps", type=int, default=10,
+                    help="steps of the 48-member ensemble (0 skips it)")
+    ap.add_argument("--graph", action="store_true", help="run the 256^3 step in graph mode")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with three 1 GiB device copies (torch.neg, float64) that calibrate HBM "
+                         "byte counters of a profiler run")
     ap.add_argument("--probe", default="zf_elastic",
                     choices=["x_r2c", "y", "z", "x_c2r", "chem", "polar", "elener", "ks_elastic",
-                             "zf_elastic", "zf_other"])
+                             "zf_elastic", "zf_other", "x_polar"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build keeps the whole grid on a single MI355X "
@@ -117,10 +174,10 @@ def main():
     phi, cons = initial_fields(w)
     n3 = args.size ** 3
     kinds = {"x_r2c": 0, "y": 1, "z": 2, "x_c2r": 3, "chem": 4, "polar": 5, "elener": 6, "ks_elastic": 7,
-             "zf_elastic": 8, "zf_other": 9}
+             "zf_elastic": 8, "zf_other": 9, "x_polar": 10}
     kind = kinds[args.probe]
 
-    r64 = run_gpu(p, phi, cons, MM_FP64, args.steps, args.warmup, kind)
+    r64 = run_gpu(p, phi, cons, MM_FP64, args.steps, args.warmup, kind, args.graph)
     cups64 = n3 * args.steps / r64["seconds"]
     n_el = [r["n_elastic"] for r in r64["rows"]]
     n_avg = sum(n_el) / len(n_el)
@@ -155,7 +212,7 @@ def main():
                             "achieved_GBs": bpc * cups64 / 1e9,
                             "frac": bpc * cups64 / 1e9 / HBM_PEAK_GBS}
     if not args.no_mixed:
-        r32 = run_gpu(p, phi, cons, MM_MIXED, args.steps, args.warmup, kind)
+        r32 = run_gpu(p, phi, cons, MM_MIXED, args.steps, args.warmup, kind, args.graph)
         cups32 = n3 * args.steps / r32["seconds"]
         l2 = {}
         for k in ("phi", "cons"):
@@ -165,7 +222,29 @@ def main():
         bpc32 = step_bytes_per_cell(3, 1, n_avg, 4)
         out["mixed"] = {"value": cups32, "speedup_vs_fp64": cups32 / cups64, "rel_l2_vs_fp64": l2,
                         "ms_per_step": 1e3 * r32["seconds"] / args.steps,
+                        "probe_kernel": args.probe,
+                        "probe_avg_launch_ms": (r32["probe_ms"] / r32["probe_launches"]
+                                                if r32["probe_launches"] else None),
                         "step_roofline_frac": bpc32 * cups32 / 1e9 / HBM_PEAK_GBS}
+    if args.ensemble_steps > 0:
+        eg = run_ensemble(args.ensemble_steps, 2, graph=True)
+        ee = run_ensemble(args.ensemble_steps, 2, graph=False)
+        out["ensemble"] = {
+            "workload": "48 members (16 each at 64^3, 96^3, 128^3), 3 tetragonal variants, eigenstrain "
+                        "U[0.005,0.03], kappa U[0.5,2], seeds 100-147, mixed FP32/FP64",
+            "steps": args.ensemble_steps,
+            "graph_cell_updates_per_s": eg["cells"] * args.ensemble_steps / eg["seconds"],
+            "graph_ms_per_step": 1e3 * eg["seconds"] / args.ensemble_steps,
+            "eager_sequential_cell_updates_per_s": ee["cells"] * args.ensemble_steps / ee["seconds"],
+            "eager_ms_per_step": 1e3 * ee["seconds"] / args.ensemble_steps,
+            "create_s": eg["create_s"]}
+    if args.calib_copy:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = {"read": a.numel() * 8, "write": b.numel() * 8}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         secs = cpu_baseline(p, phi, cons, threads)

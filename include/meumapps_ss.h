@@ -180,6 +180,7 @@ int mm_field_to_host(mm_ctx *ctx, int32_t which, double *host);
 #define MM_PROBE_KS_ELASTIC 7
 #define MM_PROBE_ZF_ELASTIC 8 /* fused z pass of the elastic iteration */
 #define MM_PROBE_ZF_OTHER 9   /* fused z passes of gradients, CH, AC */
+#define MM_PROBE_X_POLAR 10   /* fused x pass + polarization stress of the elastic iterations */
 int mm_probe(mm_ctx *ctx, int32_t kind);
 int mm_probe_read(mm_ctx *ctx, double *ms_total, int64_t *launches, double *bytes_total);
 

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fft3d.h"
@@ -17,6 +18,9 @@ namespace mm {
 template <typename T, class Op>
 int launch_zfused(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int probe_kind, double bytes,
                   int* nblk);
+template <typename T>
+int launch_zf_elastic(const Fft3& f, const OpElastic<T>& op, int v, hipStream_t s, int probe_kind, double bytes,
+                      int* nblk);
 
 // ============================================================== kernels
 
@@ -327,6 +331,38 @@ __device__ __forceinline__ void compliance_mix(const DevParams* __restrict__ P, 
 // ModElastic.f:248-327: C(x) = (sum P S)^-1, C' = CEla(:,:,2) - C(x),
 // x = cfac C' (eps + e_mean) + C(x) sum_v P_v eps0_v  with engineering
 // shears, evaluated as  cfac CEla2 b + C(x) (a - cfac b)  by one 6x6 solve.
+// Polarization stress at one point when every phase has the same compliance
+// S (ModElastic.f:262-323): the guarded mixture is S when sum P < 1 and
+// (sum P) S otherwise, so C(x) = sc C with sc = 1 / max(1, sum P),
+// C' = (1 - sc) C, and x = C (cfac (1 - sc) b + sc a), with b = strain +
+// mean strain and a = sum_v P_v eps0_v (engineering shears).
+template <int V>
+__device__ __forceinline__ void polar_homog(const DevParams* __restrict__ P, const double (&f)[V],
+                                            const double (&e)[6], double cfac, double (&x)[6]) {
+  double Pv[V], ps = 0.0;
+#pragma unroll
+  for (int v = 0; v < V; ++v) { Pv[v] = pfun(f[v]); ps = ps + Pv[v]; }
+  const double sc = ps < 1.0 ? 1.0 : 1.0 / ps;
+  const double cb = cfac * (1.0 - sc);
+  double r[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const double w = m < 3 ? 1.0 : 2.0;
+    const double bv = e[m] + w * P->e_mean[m];
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) s += Pv[v] * P->tp[v * 6 + m];
+    r[m] = cb * bv + sc * (w * s);
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    double s = 0.0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) s += P->cela[n * 6 + m] * r[n];
+    x[m] = s;
+  }
+}
+
 template <typename T, int V, int NPH, bool HOMOG>
 __global__ void __launch_bounds__(kBlock)
 k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ strain,
@@ -336,37 +372,29 @@ k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __r
   const double cfac = cfac_dev ? *cfac_dev : cfac_arg;
   bool bad = false;
   GRID_STRIDE(pt, n3) {
-    double Pv[V], ps = 0.0;
-#pragma unroll
-    for (int v = 0; v < V; ++v) { Pv[v] = pfun(ld(phi, v * n3 + pt)); ps = ps + Pv[v]; }
-    double bv[6], a[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      const double w = m < 3 ? 1.0 : 2.0;
-      bv[m] = ld(strain, m * n3 + pt) + w * P->e_mean[m];
-      double s = 0.0;
-#pragma unroll
-      for (int v = 0; v < V; ++v) s += Pv[v] * P->tp[v * 6 + m];
-      a[m] = w * s;
-    }
     if constexpr (HOMOG) {
-      // one compliance S for every phase: the guarded mixture is S when
-      // sum P < 1 and (sum P) S otherwise, so C(x) = sc C with
-      // sc = 1 / max(1, sum P), C' = (1 - sc) C, and
-      // x = C (cfac (1 - sc) b + sc a)
-      const double sc = ps < 1.0 ? 1.0 : 1.0 / ps;
-      const double cb = cfac * (1.0 - sc);
-      double r[6];
+      double f[V], e[6], x[6];
 #pragma unroll
-      for (int m = 0; m < 6; ++m) r[m] = cb * bv[m] + sc * a[m];
+      for (int v = 0; v < V; ++v) f[v] = ld(phi, v * n3 + pt);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) e[m] = ld(strain, m * n3 + pt);
+      polar_homog<V>(P, f, e, cfac, x);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) st(xpol, m * n3 + pt, x[m]);
+    } else {
+      double Pv[V], ps = 0.0;
+#pragma unroll
+      for (int v = 0; v < V; ++v) { Pv[v] = pfun(ld(phi, v * n3 + pt)); ps = ps + Pv[v]; }
+      double bv[6], a[6];
 #pragma unroll
       for (int m = 0; m < 6; ++m) {
+        const double w = m < 3 ? 1.0 : 2.0;
+        bv[m] = ld(strain, m * n3 + pt) + w * P->e_mean[m];
         double s = 0.0;
 #pragma unroll
-        for (int n = 0; n < 6; ++n) s += P->cela[n * 6 + m] * r[n];
-        st(xpol, m * n3 + pt, s);
+        for (int v = 0; v < V; ++v) s += Pv[v] * P->tp[v * 6 + m];
+        a[m] = w * s;
       }
-    } else {
       double M[6][6];
       compliance_mix<V, NPH>(P, Pv, ps, ps < 1.0, M);
       double r[6];
@@ -384,6 +412,124 @@ k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __r
     }
   }
   if (bad) *err = MM_ERR_CINV_ITER;
+}
+
+// ------------------------------------ fused x pass with the polarization stress
+// The elastic iterations after the first (ModElastic.f:241-323 between the
+// inverse transforms of the strain and the forward transforms of x): one
+// workgroup owns B rows (z, y0 .. y0+B-1) of the six strain spectra left by
+// the inverse y pass, runs their x c2r transforms into registers, forms the
+// polarization stress at every point of its rows (homogeneous moduli), runs
+// the six x r2c transforms and writes the spectra back in place.  The real
+// strain and stress never reach HBM.  Each thread keeps the same points
+// (x = 2n, 2n+1 for n = tl + TPL q) through all three phases, so the
+// pointwise phase needs no exchange.
+template <typename T, class P, int V>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
+k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, C2<T>* work, size_t nc, size_t n3,
+          const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T scale, double cfac_arg,
+          const double* __restrict__ cfac_dev, const int* __restrict__ gate) {
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  if (gated_off(gate)) return;
+  const int z = blockIdx.y, y0 = blockIdx.x * B, t = threadIdx.x;
+  // line view (transforms, pointwise) and gather view (transposed loads and stores)
+  const int bl = t / TPL, tl = t % TPL, yl = y0 + bl;
+  const int bg = t % B, r = t / B, yg = y0 + bg;
+  const IdxLineMajor idx{bl * LP};
+  C2<T> s[6][E];
+  // 1. c2r of the six strain components (imaginary parts of X_0, X_L ignored)
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const C2<T>* in = work + (size_t)m * nc;
+    C2<T> x[E + 1];
+#pragma unroll
+    for (int i = 0; i <= E; ++i) {
+      const int kx = r + TPL * i;
+      x[i] = (yg < ny && kx <= L) ? in[((size_t)z * (L + 1) + kx) * ny + yg] : mk<T>(0, 0);
+    }
+    if (m > 0) __syncthreads();
+#pragma unroll
+    for (int i = 0; i <= E; ++i) {
+      const int kx = r + TPL * i;
+      if (kx <= L) lds[bg * LP + kx + (kx >> 4)] = x[i];
+    }
+    __syncthreads();
+    C2<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int k = tl + TPL * q;
+      C2<T> xk = lds[idx(k)], xc = lds[idx(L - k)];
+      if (k == 0) { xk.y = 0; xc.y = 0; }
+      C2<T> ev = mk<T>(xk.x + xc.x, xk.y - xc.y);
+      C2<T> dd = mk<T>(xk.x - xc.x, xk.y + xc.y);
+      C2<T> w = twN[k];
+      w.y = -w.y;
+      C2<T> od = cmul<T>(dd, w);
+      v[q] = mk<T>(ev.x - od.y, ev.y + od.x);
+    }
+    __syncthreads();
+    line_fft<T, +1, P>(v, tl, lds, idx, twL);
+#pragma unroll
+    for (int q = 0; q < E; ++q) s[m][q] = v[q];
+  }
+  // 2. polarization stress at the 2E points of this thread
+  if (yl < ny) {
+    const double cfac = cfac_dev ? *cfac_dev : cfac_arg;
+    const size_t row = ((size_t)z * ny + yl) * (size_t)(2 * L);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int n = tl + TPL * q;
+      C2<T> ph[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) ph[v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[n];
+      double f[V], e[6], xo[6];
+#pragma unroll
+      for (int v = 0; v < V; ++v) f[v] = (double)ph[v].x;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) e[m] = (double)s[m][q].x;
+      polar_homog<V>(Pp, f, e, cfac, xo);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) s[m][q].x = (T)xo[m];
+#pragma unroll
+      for (int v = 0; v < V; ++v) f[v] = (double)ph[v].y;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) e[m] = (double)s[m][q].y;
+      polar_homog<V>(Pp, f, e, cfac, xo);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) s[m][q].y = (T)xo[m];
+    }
+  }
+  // 3. r2c of the six stress components, scaled, written back in place
+  const T hs = (T)0.5 * scale;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    C2<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = s[m][q];
+    __syncthreads();
+    line_fft<T, -1, P>(v, tl, lds, idx, twL);
+    if constexpr (P::NS > 1) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) lds[idx(tl + TPL * q)] = v[q];
+    __syncthreads();
+    if (yg < ny) {
+      C2<T>* out = work + (size_t)m * nc;
+#pragma unroll
+      for (int i = 0; i <= E; ++i) {
+        const int kx = r + TPL * i;
+        if (kx > L) break;
+        const int k1 = kx == L ? 0 : kx, k2 = kx == 0 ? 0 : L - kx;
+        C2<T> zk = lds[bg * LP + k1 + (k1 >> 4)];
+        C2<T> zc = lds[bg * LP + k2 + (k2 >> 4)];
+        C2<T> ev = mk<T>(zk.x + zc.x, zk.y - zc.y);
+        C2<T> od = mk<T>(zk.y + zc.y, -(zk.x - zc.x));
+        C2<T> w = twN[kx];
+        C2<T> xx = cadd<T>(ev, cmul<T>(w, od));
+        out[((size_t)z * (L + 1) + kx) * ny + yg] = cscale<T>(xx, hs);
+      }
+    }
+  }
 }
 
 // --------------------------------------------------------------- elener
@@ -713,7 +859,34 @@ template <typename T> struct Kern {
                  const int*) = nullptr;
   void (*clamp)(dim3, hipStream_t, const DevParams*, T*, const T*, double*, size_t, const int*) = nullptr;
   void (*zeroth)(dim3, hipStream_t, const DevParams*, const C2<T>*, C2<T>*, C2<T>*, size_t) = nullptr;
+  // fused x pass + polarization stress (homogeneous moduli, supported x lengths only):
+  // returns false when it has no kernel for this x length
+  bool (*xpolar)(bool launch, hipStream_t, const DevParams*, const T*, C2<T>*, size_t, size_t, const void*,
+                 const void*, int, int, int, T, double, const double*, const int*) = nullptr;
 };
+
+// plan of the fused x kernel: six components of E points stay in registers
+template <typename T, int L>
+using XPlan = std::conditional_t<sizeof(T) == 8, PlanLean4<L>, PlanLean<L>>;
+template <typename T, int L> constexpr bool xplan_ok() { return XPlan<T, L>::E <= (sizeof(T) == 8 ? 4 : 8); }
+
+template <typename T, int L, int V>
+static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* phi, C2<T>* work, size_t nc,
+                      size_t n3, const void* twL, const void* twN, int ny, int nz, T scale, double cfac,
+                      const double* cfd, const int* gate) {
+  if constexpr (!xplan_ok<T, L>()) {
+    return false;
+  } else {
+    using XP = XPlan<T, L>;
+    using TL = TileLM<T, XP>;
+    if (launch) {
+      dim3 grid((ny + TL::B - 1) / TL::B, nz);
+      hipLaunchKernelGGL((k_x_polar<T, XP, V>), grid, dim3(TL::THREADS), 0, s, P, phi, work, nc, n3,
+                         (const C2<T>*)twL, (const C2<T>*)twN, ny, scale, cfac, cfd, gate);
+    }
+    return true;
+  }
+}
 
 template <typename T, int V, int NPH, int E> struct KernImpl {
   static void chem(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, const T* c, const T* d,
@@ -740,9 +913,21 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
   static void zeroth(dim3 g, hipStream_t s, const DevParams* P, const C2<T>* a, C2<T>* b, C2<T>* c, size_t n) {
     hipLaunchKernelGGL((k_ks_zeroth<T, V>), g, dim3(kBlock), 0, s, P, a, b, c, n);
   }
+  static bool xpolar(bool launch, hipStream_t s, const DevParams* P, const T* phi, C2<T>* work, size_t nc,
+                     size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, T scale, double cfac,
+                     const double* cfd, const int* gate) {
+    switch (nx / 2) {
+#define X(L_) \
+  case L_: return go_xpolar<T, L_, V>(launch, s, P, phi, work, nc, n3, twL, twN, ny, nz, scale, cfac, cfd, gate);
+      X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+#undef X
+      default: return false;
+    }
+  }
   static Kern<T> table(bool homog) {
     Kern<T> k;
     k.chem = chem; k.ppoly = ppoly; k.clamp = clamp; k.zeroth = zeroth;
+    k.xpolar = homog ? xpolar : nullptr;
     k.polar = homog ? polar<true> : polar<false>;
     k.elener = homog ? elener<true> : elener<false>;
     return k;
@@ -793,6 +978,11 @@ struct mm_ctx {
   Probe probe;
   bool zfused = true;  // z passes with the k-space operations fused in (MM_ZFUSED=0: separate kernels)
   bool autotune = true;  // time the FFT plan variants at creation (MM_AUTOTUNE=0: defaults)
+  int zf_el_var = 0;     // launch-shape variant of the fused elastic z kernel
+  // elastic iterations after the first run the fused x pass + polarization
+  // stress (needs zfused, homogeneous moduli and a fused kernel for Nx;
+  // MM_XFUSED=0 switches it off)
+  bool xpolar = false;
   int last_niter = 0;
   double last_cfac = 1.0;
   // graph mode: the regular step (step > elastep0) recorded once and replayed
@@ -959,28 +1149,46 @@ template <typename T> int elastic_zeroth(mm_ctx* c) {
 // One fixed-point iteration (ModElastic.f:241-433) up to the six convergence
 // sums in dres[R_ERR0..]; the strain inverse transforms follow separately.
 // cfac comes from the argument (eager) or from the device control (graph).
-template <typename T> int elastic_iter(mm_ctx* c, double cfac, const double* cfac_dev) {
+// spectral: the strain of the previous iteration is held as six spectra
+// after the inverse y pass (work[0..5]) and the fused x kernel takes it from
+// there; otherwise it is read from the real strain fields.
+template <typename T> int elastic_iter(mm_ctx* c, double cfac, const double* cfac_dev, bool spectral) {
   const size_t n3 = c->n3, nc = c->nc;
   const Kern<T>& K = kern<T>(c);
   T* xpol = (T*)c->xpol;
   C2<T>* work = (C2<T>*)c->swork;
   C2<T>* uhat = (C2<T>*)c->suhat;
-  c->probe.begin(MM_PROBE_POLAR, c->s);
-  K.polar(dim3(c->nblk_pw), c->s, c->dp, (T*)c->phi, (T*)c->strain, cfac, cfac_dev, xpol, c->derr, n3, c->gate);
-  c->probe.end(MM_PROBE_POLAR, c->s, (double)n3 * sizeof(T) * (c->V + 12));
-  MM_HIP(hipGetLastError());
+  C2<T>* wsp[6];
+  for (int m = 0; m < 6; ++m) wsp[m] = work + (size_t)m * nc;
+  if (spectral) {
+    const T scale = (T)(1.0 / (double)(float)n3);
+    c->probe.begin(MM_PROBE_X_POLAR, c->s);
+    K.xpolar(true, c->s, c->dp, (const T*)c->phi, work, nc, n3, c->fft.twLx, c->fft.twNx, c->fft.nx, c->fft.ny,
+             c->fft.nz, scale, cfac, cfac_dev, c->gate);
+    c->probe.end(MM_PROBE_X_POLAR, c->s, (double)sizeof(T) * (24.0 * nc + (double)c->V * n3));
+    MM_HIP(hipGetLastError());
+    MM_TRY(c->fft.y_pass<T>((void* const*)wsp, (void* const*)wsp, 6, -1, c->s));
+  } else {
+    c->probe.begin(MM_PROBE_POLAR, c->s);
+    K.polar(dim3(c->nblk_pw), c->s, c->dp, (T*)c->phi, (T*)c->strain, cfac, cfac_dev, xpol, c->derr, n3, c->gate);
+    c->probe.end(MM_PROBE_POLAR, c->s, (double)n3 * sizeof(T) * (c->V + 12));
+    MM_HIP(hipGetLastError());
+    if (c->zfused) {
+      const T* xin[6];
+      for (int m = 0; m < 6; ++m) xin[m] = xpol + (size_t)m * n3;
+      MM_TRY(fwd_xy<T>(c, xin, wsp, 6));
+    } else {
+      MM_TRY(fwd<T>(c, xpol, n3, work, 6));
+    }
+  }
   if (c->zfused) {
-    const T* xin[6];
-    C2<T>* wsp[6];
-    for (int m = 0; m < 6; ++m) { xin[m] = xpol + (size_t)m * n3; wsp[m] = work + (size_t)m * nc; }
-    MM_TRY(fwd_xy<T>(c, xin, wsp, 6));
     OpElastic<T> op{c->dp, work, uhat, c->part, nc};
     op.gate = c->gate;
     int nblk = 0;
-    MM_TRY(launch_zfused<T>(c->fft, op, 1, c->s, MM_PROBE_ZF_ELASTIC, (double)nc * 2 * sizeof(T) * 18, &nblk));
+    MM_TRY(launch_zf_elastic<T>(c->fft, op, c->zf_el_var, c->s, MM_PROBE_ZF_ELASTIC, (double)nc * 2 * sizeof(T) * 18,
+                                &nblk));
     MM_TRY(finalize(c, nblk, 6, {0, 0, 0, 0, 0, 0}, R_ERR0));
   } else {
-    MM_TRY(fwd<T>(c, xpol, n3, work, 6));
     c->probe.begin(MM_PROBE_KS_ELASTIC, c->s);
     hipLaunchKernelGGL(k_ks_elastic<T>, dim3(c->nblk_k), dim3(kBlock), 0, c->s, c->dp, work, uhat, c->part, nc);
     c->probe.end(MM_PROBE_KS_ELASTIC, c->s, (double)nc * 2 * sizeof(T) * 18);
@@ -990,26 +1198,44 @@ template <typename T> int elastic_iter(mm_ctx* c, double cfac, const double* cfa
   return 0;
 }
 
-template <typename T> int elastic_strain_inverse(mm_ctx* c) {
+// Inverse transforms of the new strain spectra: with the fused x kernel only
+// the y pass (the strain stays spectral, returns true), else back to real.
+template <typename T> int elastic_strain_inverse(mm_ctx* c, bool* spectral) {
   const size_t n3 = c->n3, nc = c->nc;
   C2<T>* work = (C2<T>*)c->swork;
   C2<T>* wsp[6];
   T* wre[6];
   for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
+  if (c->xpolar) {
+    *spectral = true;
+    return c->fft.y_pass<T>((void* const*)wsp, (void* const*)wsp, 6, +1, c->s);
+  }
+  *spectral = false;
   if (c->zfused) return inv_yx<T>(c, wsp, wre, 6);
   return inv<T>(c, wsp, wre, 6);
 }
 
+// the real strain fields from the spectra left by elastic_strain_inverse
+template <typename T> int elastic_strain_real(mm_ctx* c) {
+  const size_t n3 = c->n3, nc = c->nc;
+  C2<T>* work = (C2<T>*)c->swork;
+  void* wsp[6];
+  T* wre[6];
+  for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
+  return c->fft.x_c2r<T>(wsp, wre, 6, c->s);
+}
+
 // eager iterations with host control, starting from the control state h
-template <typename T> int elastic_loop_eager(mm_ctx* c, Ctl& h) {
+template <typename T> int elastic_loop_eager(mm_ctx* c, Ctl& h, bool spectral) {
   while (h.active) {
-    MM_TRY(elastic_iter<T>(c, h.cfac, nullptr));
+    MM_TRY(elastic_iter<T>(c, h.cfac, nullptr, spectral));
     MM_HIP(hipMemcpyAsync(c->hres + R_ERR0, c->dres + R_ERR0, 6 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     MM_HIP(hipEventRecord(c->ev, c->s));
-    MM_TRY(elastic_strain_inverse<T>(c));
+    MM_TRY(elastic_strain_inverse<T>(c, &spectral));
     MM_HIP(hipEventSynchronize(c->ev));
     ctl_after_iter(h, c->hres + R_ERR0);
   }
+  if (spectral) MM_TRY(elastic_strain_real<T>(c));
   c->last_niter = h.total;
   c->last_cfac = h.cfac;
   return h.err;
@@ -1212,7 +1438,7 @@ template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
     if (c->step == c->prm.elastep0) MM_TRY(elastic_zeroth<T>(c));
     Ctl h;
     ctl_start(h);
-    MM_TRY(elastic_loop_eager<T>(c, h));
+    MM_TRY(elastic_loop_eager<T>(c, h, false));
   }
   MM_TRY(step_tail<T>(c, elastic));
   return step_collect(c, d);
@@ -1225,14 +1451,19 @@ template <typename T> int record_step(mm_ctx* c, double* stage) {
   MM_TRY(step_head<T>(c));
   hipLaunchKernelGGL(k_ctl_start, dim3(1), dim3(1), 0, c->s, c->ctl);
   MM_HIP(hipGetLastError());
+  bool spectral = false;
   for (int i = 0; i < c->graph_iters; ++i) {
     set_gate(c, &c->ctl->active);
-    MM_TRY(elastic_iter<T>(c, 0.0, &c->ctl->cfac));
+    MM_TRY(elastic_iter<T>(c, 0.0, &c->ctl->cfac, spectral));
     set_gate(c, nullptr);
     hipLaunchKernelGGL(k_ctl_iter, dim3(1), dim3(1), 0, c->s, c->ctl, (const double*)(c->dres + R_ERR0));
     MM_HIP(hipGetLastError());
     set_gate(c, &c->ctl->ran);
-    MM_TRY(elastic_strain_inverse<T>(c));
+    MM_TRY(elastic_strain_inverse<T>(c, &spectral));
+  }
+  if (spectral) {
+    set_gate(c, &c->ctl->done);
+    MM_TRY(elastic_strain_real<T>(c));
   }
   set_gate(c, &c->ctl->done);
   MM_TRY(step_tail<T>(c, true));
@@ -1251,7 +1482,7 @@ template <typename T> int finish_replay(mm_ctx* c, const double* st, mm_diag* d)
   std::memcpy(&h, st, sizeof(Ctl));
   if (!h.done) {
     set_gate(c, nullptr);
-    MM_TRY(elastic_loop_eager<T>(c, h));
+    MM_TRY(elastic_loop_eager<T>(c, h, c->xpolar && c->graph_iters > 0));
     MM_TRY(step_tail<T>(c, true));
     return step_collect(c, d);
   }
@@ -1467,6 +1698,18 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   {
     const char* at = std::getenv("MM_AUTOTUNE");
     if (at && at[0] == '0') c->autotune = false;
+    const char* xf = std::getenv("MM_XFUSED");
+    const bool xf_on = !(xf && xf[0] == '0');
+    if (!rc && xf_on && c->zfused) {
+      if (precision == MM_FP64)
+        c->xpolar = c->k64.xpolar && c->k64.xpolar(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr,
+                                                    p->nx, p->ny, p->nz, 0.0, 0.0, nullptr, nullptr);
+      else
+        c->xpolar = c->k32.xpolar && c->k32.xpolar(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr,
+                                                    p->nx, p->ny, p->nz, 0.0f, 0.0, nullptr, nullptr);
+    }
+    const char* zv = std::getenv("MM_ZF_ELASTIC_VARIANT");
+    if (zv) c->zf_el_var = std::max(0, std::min(kZfElasticVariants - 1, std::atoi(zv)));
   }
   if (rc) { free_all(c); delete c; return rc; }
   c->n3 = c->fft.nreal();
@@ -1590,7 +1833,7 @@ extern "C" int mm_field_to_host(mm_ctx* c, int32_t which, double* host) {
 }
 
 extern "C" int mm_probe(mm_ctx* c, int32_t kind) {
-  if (!c || kind < -1 || kind > MM_PROBE_ZF_OTHER) return MM_ERR_ARG;
+  if (!c || kind < -1 || kind > MM_PROBE_X_POLAR) return MM_ERR_ARG;
   MM_HIP(hipStreamSynchronize(c->s));
   MM_TRY(c->probe.collect());
   c->probe.reset();

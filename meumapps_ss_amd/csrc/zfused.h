@@ -53,18 +53,20 @@ __device__ __forceinline__ KVec kfull(const DevParams* __restrict__ P, const Col
   return k;
 }
 
-template <int L, bool FOUR> struct ZPlan { using type = PlanLean<L>; };
-template <int L> struct ZPlan<L, true> { using type = PlanLean4<L>; };
+// plan kind of a fused op: 0 lean (radix 8), 1 lean-4 (radix 4), 2 default
+template <int L, int K> struct ZPlan { using type = PlanLean<L>; };
+template <int L> struct ZPlan<L, 1> { using type = PlanLean4<L>; };
+template <int L> struct ZPlan<L, 2> { using type = Plan<L>; };
 
 // The twiddle pointer and the thread's line index are laundered through an
 // empty asm statement before every transform: otherwise the compiler hoists
 // all loop-invariant twiddle loads of the NIN + NOUT transforms to the kernel
 // entry and keeps them live, which costs over 60 VGPRs.
 template <typename T, int L, class Op>
-__global__ void __launch_bounds__((TileZF<T, typename ZPlan<L, Op::LEAN4>::type, Op::MAXT>::THREADS),
-                                  (TileZF<T, typename ZPlan<L, Op::LEAN4>::type, Op::MAXT>::template minw<Op::MINW>()))
+__global__ void __launch_bounds__((TileZF<T, typename ZPlan<L, Op::PLAN>::type, Op::MAXT>::THREADS),
+                                  (TileZF<T, typename ZPlan<L, Op::PLAN>::type, Op::MAXT>::template minw<Op::MINW>()))
 k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
-  using P = typename ZPlan<L, Op::LEAN4>::type;
+  using P = typename ZPlan<L, Op::PLAN>::type;
   using TL = TileZF<T, P, Op::MAXT>;
   if (gated_off(op.gate)) return;
   constexpr int TPL = P::TPL, E = P::E, B = TL::B;
@@ -124,7 +126,7 @@ k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
 // out[4j+3] = phi_hat * sum G kk.
 template <typename T> struct OpGrad {
   static constexpr int NIN = 1, NOUT = 4, MAXT = 512, MINW = 4;
-  static constexpr bool LEAN4 = false;
+  static constexpr int PLAN = 0;
   const DevParams* P;
   C2<T>* sphi;
   C2<T>* work;
@@ -180,7 +182,7 @@ template <typename T> struct OpGrad {
 // |u_old|^2 over the half spectrum; out: engineering-strain spectra (in place).
 template <typename T> struct OpElastic {
   static constexpr int NIN = 6, NOUT = 6, MAXT = 512, MINW = 1;
-  static constexpr bool LEAN4 = true;
+  static constexpr int PLAN = 1;
   const DevParams* P;
   C2<T>* work;
   C2<T>* uhat;
@@ -300,11 +302,19 @@ template <typename T> struct OpElastic {
   }
 };
 
+// launch-shape variants of the elastic op (plan kind, workgroup size cap,
+// minimum waves per SIMD), chosen at context creation by timing
+template <typename T, int PLAN_, int MAXT_, int MINW_> struct OpElasticV : OpElastic<T> {
+  static constexpr int PLAN = PLAN_, MAXT = MAXT_, MINW = MINW_;
+  OpElasticV(const OpElastic<T>& b) : OpElastic<T>(b) {}
+};
+constexpr int kZfElasticVariants = 6;
+
 // ------------------------------------------------------------ Cahn-Hilliard
 // in: work[4e] = c, work[4e+1..3] = Fx, Fy, Fz after x and y; out: c_hat new.
 template <typename T> struct OpCH {
   static constexpr int NIN = 4, NOUT = 1, MAXT = 512, MINW = 4;
-  static constexpr bool LEAN4 = true;
+  static constexpr int PLAN = 1;
   const DevParams* P;
   C2<T>* work;
   size_t nc;
@@ -359,7 +369,7 @@ template <typename T> struct OpCH {
 // out (in place): (phi_hat - dt L f_hat) / (1 - dt L sum G kk).
 template <typename T> struct OpAC {
   static constexpr int NIN = 1, NOUT = 1, MAXT = 512, MINW = 4;
-  static constexpr bool LEAN4 = false;
+  static constexpr int PLAN = 0;
   const DevParams* P;
   C2<T>* work;
   const C2<T>* sphi;

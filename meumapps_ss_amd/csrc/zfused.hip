@@ -8,7 +8,7 @@ namespace mm {
 
 template <typename T, int L, class Op>
 static int go_zf(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int* nblk) {
-  using TL = TileZF<T, typename ZPlan<L, Op::LEAN4>::type, Op::MAXT>;
+  using TL = TileZF<T, typename ZPlan<L, Op::PLAN>::type, Op::MAXT>;
   const int ncols = f.H * f.ny;
   dim3 grid((ncols + TL::B - 1) / TL::B, ngroups);
   hipLaunchKernelGGL((k_z_fused<T, L, Op>), grid, dim3(TL::THREADS), 0, s, op, (const C2<T>*)f.twNz, ncols);
@@ -30,6 +30,22 @@ int launch_zfused(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int p
   MM_HIP(hipGetLastError());
   return 0;
 }
+
+// variant v of the elastic op: (plan kind, workgroup cap, min waves per SIMD)
+template <typename T>
+int launch_zf_elastic(const Fft3& f, const OpElastic<T>& op, int v, hipStream_t s, int probe_kind, double bytes,
+                      int* nblk) {
+  switch (v) {
+    case 1: return launch_zfused<T>(f, OpElasticV<T, 1, 512, 3>(op), 1, s, probe_kind, bytes, nblk);
+    case 2: return launch_zfused<T>(f, OpElasticV<T, 1, 512, 4>(op), 1, s, probe_kind, bytes, nblk);
+    case 3: return launch_zfused<T>(f, OpElasticV<T, 1, 256, 3>(op), 1, s, probe_kind, bytes, nblk);
+    case 4: return launch_zfused<T>(f, OpElasticV<T, 1, 256, 4>(op), 1, s, probe_kind, bytes, nblk);
+    case 5: return launch_zfused<T>(f, OpElasticV<T, 1, 1024, 2>(op), 1, s, probe_kind, bytes, nblk);
+    default: return launch_zfused<T>(f, op, 1, s, probe_kind, bytes, nblk);
+  }
+}
+template int launch_zf_elastic<double>(const Fft3&, const OpElastic<double>&, int, hipStream_t, int, double, int*);
+template int launch_zf_elastic<float>(const Fft3&, const OpElastic<float>&, int, hipStream_t, int, double, int*);
 
 #define INST(T_, OP_) \
   template int launch_zfused<T_, OP_<T_>>(const Fft3&, const OP_<T_>&, int, hipStream_t, int, double, int*);
