@@ -136,6 +136,43 @@ int mm_set_step(mm_ctx *ctx, int32_t step);
  * replay.  Results are identical to eager mode. */
 int mm_set_graph(mm_ctx *ctx, int32_t on, int32_t iters);
 
+/* Adjoint of the step (no reference counterpart): reverse-mode derivative of
+ * the regular step (step > elastep0) for FP64 contexts with the same elastic
+ * moduli in every phase.  State of the derivative: lam_phi (V fields), lam_cons
+ * (E), lam_strain (6 engineering strain components), host arrays in the
+ * mm_download field order (structure of arrays, x fastest per field).
+ *
+ *   mm_adj_enable      allocate the tape and the adjoint buffers (the fused x
+ *                      path and graph mode are switched off on this context)
+ *   mm_state_bytes / mm_state_save / mm_state_load
+ *                      device-memory record of the dynamic state (phi, c, strain,
+ *                      u_hat, step and lagged check-loop values): checkpoints
+ *   mm_adj_tape_step   one forward step from the current state, recording what
+ *                      its adjoint needs (state advances as in mm_step)
+ *   mm_adj_step        pull the adjoint state back through the taped step and
+ *                      add the parameter derivatives into *grads
+ *   mm_adj_set / mm_adj_get / mm_adj_zero
+ *                      adjoint state from / to host memory
+ *
+ * grads holds dL/d of the mm_params arrays of the same names (grad_coeff
+ * entries (i,j,v) at [v*9 + j*3 + i], trans_ppt (m,v) at [v*6 + m]). */
+typedef struct mm_adj_grads {
+  double grad_coeff[MM_MAX_VAR * 9];
+  double wellph[MM_MAX_PH];
+  double cmobs[MM_MAX_EL];
+  double b_mob;
+  double trans_ppt[MM_MAX_VAR * 6];
+} mm_adj_grads;
+int mm_adj_enable(mm_ctx *ctx);
+int mm_adj_zero(mm_ctx *ctx);
+int mm_adj_set(mm_ctx *ctx, const double *lam_phi, const double *lam_cons, const double *lam_strain);
+int mm_adj_get(mm_ctx *ctx, double *lam_phi, double *lam_cons, double *lam_strain);
+int mm_state_bytes(const mm_ctx *ctx, int64_t *bytes);
+int mm_state_save(mm_ctx *ctx, void *dev_dst);
+int mm_state_load(mm_ctx *ctx, const void *dev_src);
+int mm_adj_tape_step(mm_ctx *ctx, mm_diag *diag);
+int mm_adj_step(mm_ctx *ctx, mm_adj_grads *grads);
+
 /* Ensembles (no reference counterpart; the reference runs one simulation per
  * MPI job, Multi_MEUMAPPS.f:548-1106).  n independent members, each with its
  * own parameters and grid (params[0..n-1]) in one precision.  Fields go in

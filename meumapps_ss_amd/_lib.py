@@ -9,7 +9,7 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_int32, c_size_t, c_void_p
 
-from .params import MMDiag, MMParams
+from .params import MMAdjGrads, MMDiag, MMParams
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmeumapps_ss.so")
 
@@ -62,6 +62,15 @@ def load() -> ctypes.CDLL:
         "mm_batch_member": (c_int32, [c_void_p, c_int32, POINTER(c_void_p)]),
         "mm_batch_step": (c_int32, [c_void_p, c_int32, c_void_p]),
         "mm_batch_destroy": (None, [c_void_p]),
+        "mm_adj_enable": (c_int32, [c_void_p]),
+        "mm_adj_zero": (c_int32, [c_void_p]),
+        "mm_adj_set": (c_int32, [c_void_p, D, D, D]),
+        "mm_adj_get": (c_int32, [c_void_p, D, D, D]),
+        "mm_state_bytes": (c_int32, [c_void_p, POINTER(ctypes.c_int64)]),
+        "mm_state_save": (c_int32, [c_void_p, c_void_p]),
+        "mm_state_load": (c_int32, [c_void_p, c_void_p]),
+        "mm_adj_tape_step": (c_int32, [c_void_p, POINTER(MMDiag)]),
+        "mm_adj_step": (c_int32, [c_void_p, POINTER(MMAdjGrads)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -82,4 +91,6 @@ EXPORTED = [
     "mm_fft_forward", "mm_fft_inverse", "mm_fft_spectrum_elems", "mm_ran2", "mm_ic_uniform",
     "mm_errstr", "mm_probe", "mm_probe_read", "mm_get_tuning", "mm_set_graph",
     "mm_batch_create", "mm_batch_member", "mm_batch_step", "mm_batch_destroy",
+    "mm_adj_enable", "mm_adj_zero", "mm_adj_set", "mm_adj_get", "mm_state_bytes", "mm_state_save",
+    "mm_state_load", "mm_adj_tape_step", "mm_adj_step",
 ]

@@ -951,6 +951,9 @@ template <typename T> static bool kern_table(int V, int NPH, int E, bool homog, 
 // ================================================================= context
 using namespace mm;
 
+struct mm_adj;  // adjoint tape and state (adjoint.h)
+void adj_release(mm_adj* a);
+
 struct mm_ctx {
   mm_params prm{};
   int prec = 0;
@@ -994,6 +997,7 @@ struct mm_ctx {
   int graph_iters = 4;      // elastic iterations recorded per step
   hipGraph_t g = nullptr;
   hipGraphExec_t gx = nullptr;
+  mm_adj* adj = nullptr;    // mm_adj_enable
 };
 
 namespace {
@@ -1298,7 +1302,9 @@ template <typename T> int step_head(mm_ctx* c) {
 }
 
 // tail: elener (when the elastic solve ran), Cahn-Hilliard, Allen-Cahn, clamp
-template <typename T> int step_tail(mm_ctx* c, bool elastic) {
+// rec_fd / rec_phipre (FP64 tape): copies of f_diff after elener and of phi
+// before the clamp
+template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nullptr, double* rec_phipre = nullptr) {
   const size_t n3 = c->n3, nc = c->nc;
   const int V = c->V, E = c->E;
   const Kern<T>& K = kern<T>(c);
@@ -1313,6 +1319,7 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic) {
     c->probe.end(MM_PROBE_ELENER, c->s, (double)n3 * sizeof(T) * (4 * c->V + 7));
     MM_HIP(hipGetLastError());
     MM_TRY(finalize(c, c->nblk_pw, 1, {0}, R_EL));
+    if (rec_fd) MM_HIP(hipMemcpyAsync(rec_fd, c->fdiff, (size_t)V * n3 * sizeof(T), hipMemcpyDeviceToDevice, c->s));
   } else {
     hipLaunchKernelGGL(k_fill<T>, gp, dim3(kBlock), 0, c->s, (T*)c->el_en, n3, (T)0, c->gate);
     hipLaunchKernelGGL(k_fill<T>, gp, dim3(kBlock), 0, c->s, (T*)c->elgrad, (size_t)V * n3, (T)0, c->gate);
@@ -1373,6 +1380,7 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic) {
     }
   }
   // clamp and check-loop sums (:1008-1065)
+  if (rec_phipre) MM_HIP(hipMemcpyAsync(rec_phipre, phi, (size_t)V * n3 * sizeof(T), hipMemcpyDeviceToDevice, c->s));
   K.clamp(gp, c->s, c->dp, phi, (T*)c->cons, c->part, n3, c->gate);
   MM_HIP(hipGetLastError());
   FinalizeOps fo;
@@ -1521,6 +1529,8 @@ int graph_build(mm_ctx* c, hipStream_t cap) {
 }
 
 void free_all(mm_ctx* c) {
+  adj_release(c->adj);
+  c->adj = nullptr;
   if (c->gx) (void)hipGraphExecDestroy(c->gx);
   if (c->g) (void)hipGraphDestroy(c->g);
   c->gx = nullptr;
@@ -1997,3 +2007,5 @@ extern "C" int mm_batch_step(mm_batch* b, int32_t nsteps, mm_diag* diag) {
   }
   return 0;
 }
+
+#include "adjoint.h"

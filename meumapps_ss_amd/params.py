@@ -94,6 +94,18 @@ class MMDiag(ctypes.Structure):
         return d
 
 
+class MMAdjGrads(ctypes.Structure):
+    """ctypes mirror of ``mm_adj_grads``: dL/d of the mm_params arrays."""
+
+    _fields_ = [
+        ("grad_coeff", ctypes.c_double * (MM_MAX_VAR * 9)),
+        ("wellph", ctypes.c_double * MM_MAX_PH),
+        ("cmobs", ctypes.c_double * MM_MAX_EL),
+        ("b_mob", ctypes.c_double),
+        ("trans_ppt", ctypes.c_double * (MM_MAX_VAR * 6)),
+    ]
+
+
 @dataclass
 class PhaseInput:
     """One phase block of Input.txt (Multi_MEUMAPPS.f:178-204)."""

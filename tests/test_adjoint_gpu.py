@@ -1,0 +1,106 @@
+"""Hand-written adjoint of the step against central finite differences of the
+GPU forward (FP64, step 1e-6 of each scalar, unit-scale direction for the
+initial order parameters; bar 1e-5 relative), checkpoint-interval invariance
+and the torch.autograd.Function wrapper."""
+import numpy as np
+import pytest
+
+from meumapps_ss_amd.adjoint import AdjointRun, Problem, build_jacobian, chain, spectral_steps
+from meumapps_ss_amd.configs import model_inputs
+from meumapps_ss_amd.params import setup
+from parity_cases import smooth_field
+
+pytestmark = pytest.mark.gpu
+
+THETA = {"kappa": 1.0, "M": 1.0, "L": 1.0, "eps": 0.02}
+TOL = 1e-5
+
+
+def builder(n, numph, nvar):
+    def build(th):
+        m = th["eps"]
+        inp = model_inputs(n, nvar, (m, 0.5 * m, 0.5 * m), 0.005, kappa=th["kappa"], mob_c=th["M"],
+                           mob_eta=th["L"], numph=numph)
+        return setup(inp)
+    return build
+
+
+def fields(n, V, seed):
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    phi0 = np.stack([smooth_field(rng, shape, -0.1, 1.0) for _ in range(V)])
+    cons0 = smooth_field(rng, shape, 0.35, 0.55)[None]
+    target = np.stack([smooth_field(rng, shape, 0.0, 1.0) for _ in range(V)])
+    dphi = np.stack([smooth_field(rng, shape, -1.0, 1.0) for _ in range(V)])
+    return phi0, cons0, target, dphi
+
+
+def loss(build, th, phi0, cons0, target, nsteps):
+    run = AdjointRun(build(th))
+    phiT, _, _ = run.forward(phi0, cons0, nsteps, nsteps)
+    run.close()
+    return float(np.mean((phiT - target) ** 2))
+
+
+@pytest.mark.parametrize("n,numph,nvar", [(16, 3, 1), (16, 2, 3)])
+def test_directional_derivatives_match_central_differences(gpu_lib, n, numph, nvar):
+    build = builder(n, numph, nvar)
+    V = nvar * (numph - 1)
+    nsteps, every = 6, 4
+    phi0, cons0, target, dphi = fields(n, V, 11 + V)
+    run = AdjointRun(build(THETA))
+    phiT, ck, rows = run.forward(phi0, cons0, nsteps, every)
+    assert all(r["n_elastic"] >= 4 for r in rows)
+    lam = 2.0 * (phiT - target) / phiT.size
+    gphi, _, g = run.backward(lam, ck, nsteps, every)
+    run.close()
+    dth = chain(g, build_jacobian(build, THETA))
+    h = 1e-6
+    # initial order parameters along dphi
+    fd = (loss(build, THETA, phi0 + h * dphi, cons0, target, nsteps) -
+          loss(build, THETA, phi0 - h * dphi, cons0, target, nsteps)) / (2 * h)
+    ad = float(np.sum(gphi * dphi))
+    assert abs(ad - fd) <= TOL * abs(fd), f"phi0: adjoint {ad:.10e} fd {fd:.10e}"
+    for name, t in THETA.items():
+        hh = h * abs(t)
+        up = dict(THETA, **{name: t + hh})
+        dn = dict(THETA, **{name: t - hh})
+        fd = (loss(build, up, phi0, cons0, target, nsteps) - loss(build, dn, phi0, cons0, target, nsteps)) / (2 * hh)
+        assert abs(dth[name] - fd) <= TOL * max(abs(fd), 1e-12), f"{name}: adjoint {dth[name]:.10e} fd {fd:.10e}"
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(gpu_lib):
+    build = builder(16, 3, 1)
+    phi0, cons0, target, _ = fields(16, 2, 5)
+    out = []
+    for every in (1, 3, 7):
+        run = AdjointRun(build(THETA))
+        phiT, ck, _ = run.forward(phi0, cons0, 7, every)
+        gphi, gc, g = run.backward(2.0 * (phiT - target) / phiT.size, ck, 7, every)
+        run.close()
+        out.append((gphi, gc, np.array(list(g.grad_coeff) + list(g.wellph) + [g.b_mob] + list(g.trans_ppt))))
+    for o in out[1:]:
+        for a, b in zip(o, out[0]):
+            assert np.array_equal(a, b)
+
+
+def test_autograd_function(gpu_lib):
+    import torch
+
+    build = builder(16, 3, 1)
+    phi0, cons0, target, dphi = fields(16, 2, 8)
+    pr = Problem(build=build, names=list(THETA), cons0=cons0, nsteps=5, every=2)
+    x = torch.tensor(phi0, requires_grad=True)
+    th = torch.tensor([THETA[k] for k in pr.names], dtype=torch.float64, requires_grad=True)
+    phiT = spectral_steps(x, th, pr)
+    L = torch.mean((phiT - torch.tensor(target)) ** 2)
+    L.backward()
+    assert abs(L.item() - loss(build, THETA, phi0, cons0, target, 5)) < 1e-15
+    h = 1e-6
+    fd = (loss(build, THETA, phi0 + h * dphi, cons0, target, 5) -
+          loss(build, THETA, phi0 - h * dphi, cons0, target, 5)) / (2 * h)
+    assert abs(float((x.grad.numpy() * dphi).sum()) - fd) <= TOL * abs(fd)
+    k = pr.names.index("L")
+    up, dn = dict(THETA, L=1.0 + h), dict(THETA, L=1.0 - h)
+    fd = (loss(build, up, phi0, cons0, target, 5) - loss(build, dn, phi0, cons0, target, 5)) / (2 * h)
+    assert abs(th.grad[k].item() - fd) <= TOL * abs(fd)

@@ -98,3 +98,29 @@ def test_ensemble_workload_definition():
         p = w.params()
         assert p.nx == w.inputs.nx and p.vartot == 3
     assert 0.5 <= min(ks) and max(ks) <= 2.0 and len(set(ks)) == 48 and len(set(ms)) == 48
+
+
+def test_build_jacobian_of_linear_setup():
+    from meumapps_ss_amd.adjoint import build_jacobian, chain
+    from meumapps_ss_amd.params import MMAdjGrads
+
+    def build(th):
+        m = th["eps"]
+        return setup(model_inputs(16, 1, (m, 0.5 * m, 0.5 * m), 0.005, kappa=th["kappa"], mob_c=th["M"],
+                                  mob_eta=th["L"], numph=3))
+
+    th = {"kappa": 1.3, "M": 0.7, "L": 1.1, "eps": 0.02}
+    jac = build_jacobian(build, th)
+    p = build(th)
+    # grad_coeff = kappa * I per variant; wells scale with kappa
+    gk = jac["kappa"]["grad_coeff"].reshape(16, 3, 3)
+    assert np.allclose(gk[:2], np.eye(3)) and np.abs(gk[2:]).max() == 0
+    assert np.allclose(jac["kappa"]["wellph"][:3], np.array(p.wellph[:3]) / 1.3, rtol=1e-12)
+    assert np.allclose(jac["M"]["cmobs"][0], 1.0) and np.allclose(jac["L"]["b_mob"], 1.0)
+    te = jac["eps"]["trans_ppt"].reshape(16, 6)
+    assert np.allclose(te[:2, :3], [[1.0, 0.5, 0.5]] * 2)
+    # chain is the dot product with the raw derivatives
+    g = MMAdjGrads()
+    g.b_mob = 2.0
+    g.cmobs[0] = 3.0
+    assert abs(chain(g, jac)["L"] - 2.0) < 1e-9 and abs(chain(g, jac)["M"] - 3.0) < 1e-9
