@@ -203,8 +203,16 @@ def main():
         avg_ms = r64["probe_ms"] / r64["probe_launches"]
         bytes_per_launch = r64["probe_bytes"] / r64["probe_launches"]
         ach = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = None
+        tj = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+        if args.probe == "zf_elastic" and args.size == 256 and os.path.exists(tj):
+            # HBM bytes per launch of this kernel on this workload from the committed
+            # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
+            t = json.load(open(tj)).get("k_z_fused<double,256,OpElastic>")
+            if t:
+                traffic = t["fetch_bytes"] + t["write_bytes"]
         out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": args.probe,
+                           "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": args.probe,
                            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
                            "launches": r64["probe_launches"]}
     bpc = step_bytes_per_cell(3, 1, n_avg, 8)
