@@ -39,6 +39,15 @@ module mm_abi
      real(c_double) :: cfac
   end type mm_diag
 
+  ! dL/d of the mm_params arrays of the same names (adjoint)
+  type, bind(c) :: mm_adj_grads
+     real(c_double) :: grad_coeff(3, 3, MM_MAX_VAR)
+     real(c_double) :: wellph(MM_MAX_PH)
+     real(c_double) :: cmobs(MM_MAX_EL)
+     real(c_double) :: b_mob
+     real(c_double) :: trans_ppt(6, MM_MAX_VAR)
+  end type mm_adj_grads
+
   interface
      integer(c_int) function mm_create(p, precision, ctx) bind(c, name="mm_create")
        import :: c_int, c_ptr, mm_params
@@ -96,6 +105,90 @@ module mm_abi
        real(c_double), intent(in) :: cav(*)
        real(c_double), intent(out) :: phi(*), cons(*)
      end function mm_ic_uniform
+
+     integer(c_int) function mm_set_graph(ctx, on, iters) bind(c, name="mm_set_graph")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on, iters
+     end function mm_set_graph
+
+     ! ensembles: params(n); diag(n, nsteps)
+     integer(c_int) function mm_batch_create(n, params, precision, batch) bind(c, name="mm_batch_create")
+       import :: c_int, c_ptr, c_int32_t, mm_params
+       integer(c_int32_t), value :: n
+       type(mm_params), intent(in) :: params(*)
+       integer(c_int32_t), value :: precision
+       type(c_ptr), intent(out) :: batch
+     end function mm_batch_create
+
+     integer(c_int) function mm_batch_member(batch, i, ctx) bind(c, name="mm_batch_member")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: batch
+       integer(c_int32_t), value :: i
+       type(c_ptr), intent(out) :: ctx
+     end function mm_batch_member
+
+     integer(c_int) function mm_batch_step(batch, nsteps, diag) bind(c, name="mm_batch_step")
+       import :: c_int, c_ptr, c_int32_t, mm_diag
+       type(c_ptr), value :: batch
+       integer(c_int32_t), value :: nsteps
+       type(mm_diag), intent(out) :: diag(*)
+     end function mm_batch_step
+
+     subroutine mm_batch_destroy(batch) bind(c, name="mm_batch_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: batch
+     end subroutine mm_batch_destroy
+
+     ! adjoint and device checkpoints
+     integer(c_int) function mm_adj_enable(ctx) bind(c, name="mm_adj_enable")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function mm_adj_enable
+
+     integer(c_int) function mm_adj_zero(ctx) bind(c, name="mm_adj_zero")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function mm_adj_zero
+
+     ! lam_phi(Nx,Ny,Nz,V), lam_cons(Nx,Ny,Nz,E), lam_strain(Nx,Ny,Nz,6); c_null_ptr skips
+     integer(c_int) function mm_adj_set(ctx, lam_phi, lam_cons, lam_strain) bind(c, name="mm_adj_set")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, lam_phi, lam_cons, lam_strain
+     end function mm_adj_set
+
+     integer(c_int) function mm_adj_get(ctx, lam_phi, lam_cons, lam_strain) bind(c, name="mm_adj_get")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, lam_phi, lam_cons, lam_strain
+     end function mm_adj_get
+
+     integer(c_int) function mm_state_bytes(ctx, bytes) bind(c, name="mm_state_bytes")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: bytes
+     end function mm_state_bytes
+
+     integer(c_int) function mm_state_save(ctx, dev_dst) bind(c, name="mm_state_save")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, dev_dst
+     end function mm_state_save
+
+     integer(c_int) function mm_state_load(ctx, dev_src) bind(c, name="mm_state_load")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, dev_src
+     end function mm_state_load
+
+     integer(c_int) function mm_adj_tape_step(ctx, diag) bind(c, name="mm_adj_tape_step")
+       import :: c_int, c_ptr, mm_diag
+       type(c_ptr), value :: ctx
+       type(mm_diag), intent(out) :: diag
+     end function mm_adj_tape_step
+
+     integer(c_int) function mm_adj_step(ctx, grads) bind(c, name="mm_adj_step")
+       import :: c_int, c_ptr, mm_adj_grads
+       type(c_ptr), value :: ctx
+       type(mm_adj_grads), intent(inout) :: grads
+     end function mm_adj_step
   end interface
 
 contains

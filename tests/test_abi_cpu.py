@@ -110,3 +110,29 @@ def test_create_rejects_bad_parameters_before_touching_a_device():
     ctx = ctypes.c_void_p()
     assert lib.mm_create(ctypes.byref(p), 0, ctypes.byref(ctx)) == 20001
     assert lib.mm_create(ctypes.byref(p), 7, ctypes.byref(ctx)) == 20001
+
+
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not in this image")
+def test_fortran_binding_compiles_and_types_match(tmp_path):
+    """The ISO_C_BINDING module compiles and its bind(c) types have the byte
+    sizes of the C structs."""
+    import subprocess
+
+    from meumapps_ss_amd.params import MMAdjGrads
+
+    src = tmp_path / "sizes.f90"
+    src.write_text(
+        "program sizes\n  use mm_abi\n  type(mm_params) :: p\n  type(mm_diag) :: d\n"
+        "  type(mm_adj_grads) :: g\n  print '(3I8)', c_sizeof(p), c_sizeof(d), c_sizeof(g)\nend program sizes\n")
+    mod = os.path.join(ROOT, "meumapps_ss_amd", "fortran", "mm_abi.f90")
+    exe = tmp_path / "sizes"
+    libdir = os.path.join(ROOT, "meumapps_ss_amd")
+    _lib.load()  # the library is built
+    subprocess.run([FLANG, "-J", str(tmp_path), mod, str(src), "-o", str(exe), "-L", libdir, "-lmeumapps_ss",
+                    "-Wl,-rpath," + libdir, "-Wl,--allow-shlib-undefined"], check=True, cwd=tmp_path,
+                   capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert [int(x) for x in out] == [ctypes.sizeof(MMParams), ctypes.sizeof(MMDiag), ctypes.sizeof(MMAdjGrads)]
