@@ -57,6 +57,12 @@ struct Fft3 {
   int x_c2r(void* const* in, T* const* out, int nf, hipStream_t s) const;
   template <typename T>
   int y_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t s) const;
+  // y pass with the layout change of the fused path: mode 1 S -> Z, mode 2 Z -> S (out of place)
+  template <typename T>
+  int y_pass_t(void* const* in, void* const* out, int nf, int dir, int mode, hipStream_t s) const;
+  // re-order nf spectra between S and Z (mode 1 S -> Z, mode 2 Z -> S), out of place
+  template <typename T>
+  int perm_sz(const void* in, void* out, int nf, int mode, hipStream_t s) const;
   template <typename T>
   int z_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t s) const;
 };

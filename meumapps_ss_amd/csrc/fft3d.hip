@@ -239,6 +239,46 @@ int Fft3::y_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t
   return 0;
 }
 
+template <typename T, class P>
+static void go_yt(const Batch<T>& b, int nf, const void* tw, int nz, int H, int dir, int mode, hipStream_t s) {
+  using TL = TileLM<T, P>;
+  dim3 grid((nz * H + TL::B - 1) / TL::B, 1, nf);
+  const C2<T>* t = (const C2<T>*)tw;
+  if (dir < 0 && mode == 1) hipLaunchKernelGGL((k_y_pass_t<T, P, -1, 1>), grid, dim3(TL::THREADS), 0, s, b, t, nz, H);
+  else if (dir < 0) hipLaunchKernelGGL((k_y_pass_t<T, P, -1, 2>), grid, dim3(TL::THREADS), 0, s, b, t, nz, H);
+  else if (mode == 1) hipLaunchKernelGGL((k_y_pass_t<T, P, 1, 1>), grid, dim3(TL::THREADS), 0, s, b, t, nz, H);
+  else hipLaunchKernelGGL((k_y_pass_t<T, P, 1, 2>), grid, dim3(TL::THREADS), 0, s, b, t, nz, H);
+}
+
+template <typename T>
+int Fft3::y_pass_t(void* const* in, void* const* out, int nf, int dir, int mode, hipStream_t s) const {
+  if (nf < 1 || nf > kMaxBatch || (mode != 1 && mode != 2) || ny % kTW) return MM_ERR_ARG;
+  Batch<T> b = make_batch<T>((const void* const*)in, out, nf, gate);
+  if (probe) probe->begin(MM_PROBE_Y, s);
+  switch (ny) {
+#define X(L_) case L_: MM_VARIANTS(go_yt, L_, var[2], b, nf, twNy, nz, H, dir, mode, s); break;
+    MM_NLENS(X)
+#undef X
+    default: return MM_ERR_ARG;
+  }
+  if (probe) probe->end(MM_PROBE_Y, s, (double)nf * 2.0 * ncplx() * 2 * sizeof(T));
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int Fft3::perm_sz(const void* in, void* out, int nf, int mode, hipStream_t s) const {
+  if (nf < 1 || (mode != 1 && mode != 2) || ny % kTW) return MM_ERR_ARG;
+  const size_t n = ncplx() * nf;
+  const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
+  if (mode == 1)
+    hipLaunchKernelGGL((k_perm_sz<T, 1>), dim3(grid), dim3(256), 0, s, (const C2<T>*)in, (C2<T>*)out, nf, ny, nz, H);
+  else
+    hipLaunchKernelGGL((k_perm_sz<T, 2>), dim3(grid), dim3(256), 0, s, (const C2<T>*)in, (C2<T>*)out, nf, ny, nz, H);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
 template <typename T>
 int Fft3::z_pass(void* const* in, void* const* out, int nf, int dir, hipStream_t s) const {
   if (nf < 1 || nf > kMaxBatch) return MM_ERR_ARG;
@@ -283,6 +323,10 @@ template int Fft3::x_c2r<float>(void* const*, float* const*, int, hipStream_t) c
 template int Fft3::y_pass<double>(void* const*, void* const*, int, int, hipStream_t) const;
 template int Fft3::y_pass<float>(void* const*, void* const*, int, int, hipStream_t) const;
 template int Fft3::z_pass<double>(void* const*, void* const*, int, int, hipStream_t) const;
+template int Fft3::y_pass_t<double>(void* const*, void* const*, int, int, int, hipStream_t) const;
+template int Fft3::y_pass_t<float>(void* const*, void* const*, int, int, int, hipStream_t) const;
+template int Fft3::perm_sz<double>(const void*, void*, int, int, hipStream_t) const;
+template int Fft3::perm_sz<float>(const void*, void*, int, int, hipStream_t) const;
 template int Fft3::z_pass<float>(void* const*, void* const*, int, int, hipStream_t) const;
 
 // ---------------------------------------------------------------- autotuner

@@ -185,6 +185,78 @@ k_y_pass(Batch<T> bt, const C2<T>* __restrict__ twL, int nlines) {
   }
 }
 
+// --------------------------------------------- column-tiled spectrum layout
+// The fused z passes read the spectra in a tiled order instead of S: the
+// columns (kx, ky) are grouped in tiles of kTW consecutive ky, and a tile is
+// stored contiguously over kz,
+//   Z[(kx*Ny/kTW + ky/kTW)][kz][ky % kTW],
+// so a workgroup that owns whole tiles reads and writes contiguous blocks
+// (Nz*kTW elements) however few columns it holds.  The y pass converts:
+// forward S -> Z, inverse Z -> S (out of place).  Ny must divide by kTW.
+constexpr int kTW = 4;
+__host__ __device__ __forceinline__ size_t zidx(int kz, int kx, int ky, int ny, int nz) {
+  return (((size_t)kx * (ny / kTW) + ky / kTW) * nz + kz) * kTW + (ky % kTW);
+}
+// the same for a column id col = kx*Ny + ky
+__host__ __device__ __forceinline__ size_t zcol(int col, int kz, int nz) {
+  return ((size_t)(col / kTW) * nz + kz) * kTW + (col % kTW);
+}
+
+// y pass between the layouts: MODE 1 reads S lines and writes Z, MODE 2 reads
+// Z and writes S lines.  Lines run kz fastest (line = kx*Nz + kz) so that the
+// lines of a workgroup fill whole kTW-runs of the Z side.
+template <typename T, class P, int DIR, int MODE>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS))
+k_y_pass_t(Batch<T> bt, const C2<T>* __restrict__ twL, int nz, int H) {
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  if (gated_off(bt.gate)) return;
+  const int f = blockIdx.z, t = threadIdx.x;
+  const int b = t / TPL, tl = t % TPL, line = blockIdx.x * B + b;
+  const bool ok = line < nz * H;
+  const int kx = ok ? line / nz : 0, kz = ok ? line % nz : 0;
+  const C2<T>* __restrict__ in = (const C2<T>*)bt.in[f];
+  C2<T>* __restrict__ out = (C2<T>*)bt.out[f];
+  const size_t srow = ((size_t)kz * H + kx) * L;
+  C2<T> v[E];
+  if (ok) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int ky = tl + TPL * q;
+      v[q] = MODE == 1 ? in[srow + ky] : in[zidx(kz, kx, ky, L, nz)];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = mk<T>(0, 0);
+  }
+  IdxLineMajor idx{b * LP};
+  line_fft<T, DIR, P>(v, tl, lds, idx, twL);
+  if (ok) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int ky = tl + TPL * q;
+      if (MODE == 1) out[zidx(kz, kx, ky, L, nz)] = v[q];
+      else out[srow + ky] = v[q];
+    }
+  }
+}
+
+// re-order nf spectra S -> Z (MODE 1) or Z -> S (MODE 2), out of place
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_perm_sz(const C2<T>* __restrict__ in, C2<T>* __restrict__ out, int nf,
+                                                 int ny, int nz, int H) {
+  const size_t nc = (size_t)H * ny * nz;
+  for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < nc * nf; i += (size_t)gridDim.x * 256) {
+    const size_t f = i / nc, r = i % nc;
+    const int ky = (int)(r % ny);
+    const size_t l = r / ny;
+    const int kx = (int)(l % H), kz = (int)(l / H);
+    const size_t z = f * nc + zidx(kz, kx, ky, ny, nz);
+    if (MODE == 1) out[z] = in[i];
+    else out[i] = in[z];
+  }
+}
+
 // ----------------------------------------------------- z pass (strided lines)
 // grid (ceil(ncols/B), 1, nfields); column c = kx*Ny + ky, stride ncols
 template <typename T, class P, int DIR>

@@ -969,6 +969,9 @@ struct mm_ctx {
   void *phi = nullptr, *cons = nullptr, *gphi = nullptr, *kdel = nullptr, *fdiff = nullptr, *flux = nullptr,
        *strain = nullptr, *xpol = nullptr, *el_en = nullptr, *elgrad = nullptr;
   void *sphi = nullptr, *suhat = nullptr, *swork = nullptr;
+  // fused path: spectra between the x and y passes (S layout), kStageN of them;
+  // the y pass moves them to and from the column-tiled layout of the z kernels
+  void* sstage = nullptr;
   int nwork = 0, nxpol = 0;
   double* part = nullptr;
   double* dres = nullptr;
@@ -1003,6 +1006,7 @@ struct mm_ctx {
 namespace {
 
 constexpr int kMaxPartSlots = 8;
+constexpr int kStageN = 6;
 constexpr int kMaxBlocks = 2048;      // grid of the pointwise and k-space kernels
 constexpr int kMaxPartBlocks = 65536; // largest grid that writes partial sums
 
@@ -1112,22 +1116,37 @@ template <typename T> int inv(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf
   return 0;
 }
 
-// x and y passes only (the z pass is fused with a k-space operation)
+// x and y passes only (the z pass is fused with a k-space operation): real
+// fields -> S spectra in the stage -> column-tiled spectra out[]
 template <typename T> int fwd_xy(mm_ctx* c, const T* const* in, C2<T>* const* out, int nf) {
   const T scale = (T)(1.0 / (double)(float)c->n3);
-  for (int f0 = 0; f0 < nf; f0 += kMaxBatch) {
-    const int g = std::min(kMaxBatch, nf - f0);
-    MM_TRY(c->fft.x_r2c<T>(in + f0, (void* const*)(out + f0), g, c->s, scale));
-    MM_TRY(c->fft.y_pass<T>((void* const*)(out + f0), (void* const*)(out + f0), g, -1, c->s));
+  C2<T>* stg = (C2<T>*)c->sstage;
+  void* sp[kStageN];
+  for (int j = 0; j < kStageN; ++j) sp[j] = stg + (size_t)j * c->nc;
+  for (int f0 = 0; f0 < nf; f0 += kStageN) {
+    const int g = std::min(kStageN, nf - f0);
+    MM_TRY(c->fft.x_r2c<T>(in + f0, sp, g, c->s, scale));
+    MM_TRY(c->fft.y_pass_t<T>(sp, (void* const*)(out + f0), g, -1, 1, c->s));
   }
   return 0;
 }
+// column-tiled spectra sp[] -> S spectra in the stage -> real fields out[]
 template <typename T> int inv_yx(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf) {
-  for (int f0 = 0; f0 < nf; f0 += kMaxBatch) {
-    const int g = std::min(kMaxBatch, nf - f0);
-    MM_TRY(c->fft.y_pass<T>((void* const*)(sp + f0), (void* const*)(sp + f0), g, +1, c->s));
-    MM_TRY(c->fft.x_c2r<T>((void* const*)(sp + f0), out + f0, g, c->s));
+  C2<T>* stg = (C2<T>*)c->sstage;
+  void* st[kStageN];
+  for (int j = 0; j < kStageN; ++j) st[j] = stg + (size_t)j * c->nc;
+  for (int f0 = 0; f0 < nf; f0 += kStageN) {
+    const int g = std::min(kStageN, nf - f0);
+    MM_TRY(c->fft.y_pass_t<T>((void* const*)(sp + f0), st, g, +1, 2, c->s));
+    MM_TRY(c->fft.x_c2r<T>(st, out + f0, g, c->s));
   }
+  return 0;
+}
+// u_hat between the full-transform S layout and the tiled layout of the fused path
+template <typename T> int uhat_s_to_z(mm_ctx* c, C2<T>* scratch3) {
+  if (!c->zfused) return 0;
+  MM_TRY(c->fft.perm_sz<T>(c->suhat, scratch3, 3, 1, c->s));
+  MM_HIP(hipMemcpyAsync(c->suhat, scratch3, 3 * c->nc * sizeof(C2<T>), hipMemcpyDeviceToDevice, c->s));
   return 0;
 }
 
@@ -1147,6 +1166,7 @@ template <typename T> int elastic_zeroth(mm_ctx* c) {
   K.zeroth(dim3(c->nblk_k), c->s, c->dp, phat, (C2<T>*)c->suhat, work, nc);
   MM_HIP(hipGetLastError());
   MM_TRY(inv<T>(c, wsp, wre, 6));
+  MM_TRY(uhat_s_to_z<T>(c, work));
   return 0;
 }
 
@@ -1167,11 +1187,14 @@ template <typename T> int elastic_iter(mm_ctx* c, double cfac, const double* cfa
   if (spectral) {
     const T scale = (T)(1.0 / (double)(float)n3);
     c->probe.begin(MM_PROBE_X_POLAR, c->s);
-    K.xpolar(true, c->s, c->dp, (const T*)c->phi, work, nc, n3, c->fft.twLx, c->fft.twNx, c->fft.nx, c->fft.ny,
+    C2<T>* stg = (C2<T>*)c->sstage;
+    K.xpolar(true, c->s, c->dp, (const T*)c->phi, stg, nc, n3, c->fft.twLx, c->fft.twNx, c->fft.nx, c->fft.ny,
              c->fft.nz, scale, cfac, cfac_dev, c->gate);
     c->probe.end(MM_PROBE_X_POLAR, c->s, (double)sizeof(T) * (24.0 * nc + (double)c->V * n3));
     MM_HIP(hipGetLastError());
-    MM_TRY(c->fft.y_pass<T>((void* const*)wsp, (void* const*)wsp, 6, -1, c->s));
+    void* sp[6];
+    for (int m = 0; m < 6; ++m) sp[m] = stg + (size_t)m * nc;
+    MM_TRY(c->fft.y_pass_t<T>(sp, (void* const*)wsp, 6, -1, 1, c->s));
   } else {
     c->probe.begin(MM_PROBE_POLAR, c->s);
     K.polar(dim3(c->nblk_pw), c->s, c->dp, (T*)c->phi, (T*)c->strain, cfac, cfac_dev, xpol, c->derr, n3, c->gate);
@@ -1211,8 +1234,12 @@ template <typename T> int elastic_strain_inverse(mm_ctx* c, bool* spectral) {
   T* wre[6];
   for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
   if (c->xpolar) {
+    // the strain stays as S spectra in the stage for the fused x kernel
     *spectral = true;
-    return c->fft.y_pass<T>((void* const*)wsp, (void* const*)wsp, 6, +1, c->s);
+    C2<T>* stg = (C2<T>*)c->sstage;
+    void* sp[6];
+    for (int m = 0; m < 6; ++m) sp[m] = stg + (size_t)m * nc;
+    return c->fft.y_pass_t<T>((void* const*)wsp, sp, 6, +1, 2, c->s);
   }
   *spectral = false;
   if (c->zfused) return inv_yx<T>(c, wsp, wre, 6);
@@ -1222,10 +1249,10 @@ template <typename T> int elastic_strain_inverse(mm_ctx* c, bool* spectral) {
 // the real strain fields from the spectra left by elastic_strain_inverse
 template <typename T> int elastic_strain_real(mm_ctx* c) {
   const size_t n3 = c->n3, nc = c->nc;
-  C2<T>* work = (C2<T>*)c->swork;
+  C2<T>* stg = (C2<T>*)c->sstage;
   void* wsp[6];
   T* wre[6];
-  for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
+  for (int m = 0; m < 6; ++m) { wsp[m] = stg + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
   return c->fft.x_c2r<T>(wsp, wre, 6, c->s);
 }
 
@@ -1537,7 +1564,7 @@ void free_all(mm_ctx* c) {
   c->g = nullptr;
   void* bufs[] = {c->phi, c->cons, c->gphi, c->kdel, c->fdiff, c->flux, c->strain, c->xpol, c->el_en,
                   c->elgrad, c->sphi, c->suhat, c->swork, c->part, c->dres, c->derr, c->dp, c->ctl,
-                  c->dstage};
+                  c->dstage, c->sstage};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
@@ -1617,6 +1644,7 @@ int upload(mm_ctx* c, const double* phi, const double* cons, const double* el_en
     std::vector<double> h(u, u + 3 * n3);
     MM_TRY(put_field<T>(c, c->xpol, h));
     MM_TRY(fwd<T>(c, (T*)c->xpol, n3, (C2<T>*)c->suhat, 3));
+    MM_TRY(uhat_s_to_z<T>(c, (C2<T>*)c->swork));
     MM_HIP(hipStreamSynchronize(c->s));
   }
   return 0;
@@ -1647,6 +1675,11 @@ int download(mm_ctx* c, double* phi, double* cons, double* el_en, double* u, dou
   C2<T>* work = (C2<T>*)c->swork;
   C2<T>* uh = (C2<T>*)c->suhat;
   T* scratch = (T*)c->xpol;
+  if (c->zfused && (u || gradu)) {
+    // S-layout copy of u_hat in work[3..5]
+    uh = work + 3 * nc;
+    MM_TRY(c->fft.perm_sz<T>(c->suhat, uh, 3, 2, c->s));
+  }
   if (u) {
     MM_HIP(hipMemcpyAsync(work, uh, 3 * nc * sizeof(C2<T>), hipMemcpyDeviceToDevice, c->s));
     C2<T>* sp[3] = {work, work + nc, work + 2 * nc};
@@ -1745,6 +1778,7 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   rc = rc ? rc : dalloc(&c->sphi, V * nc * cb);
   rc = rc ? rc : dalloc(&c->suhat, 3 * nc * cb);
   rc = rc ? rc : dalloc(&c->swork, c->nwork * nc * cb);
+  if (c->zfused) rc = rc ? rc : dalloc(&c->sstage, (size_t)kStageN * nc * cb);
   rc = rc ? rc : dalloc((void**)&c->part, (size_t)kMaxPartSlots * kMaxPartBlocks * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->dres, 64 * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->derr, 16);

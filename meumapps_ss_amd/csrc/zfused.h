@@ -58,10 +58,11 @@ template <int L, int K> struct ZPlan { using type = PlanLean<L>; };
 template <int L> struct ZPlan<L, 1> { using type = PlanLean4<L>; };
 template <int L> struct ZPlan<L, 2> { using type = Plan<L>; };
 
-// The twiddle pointer and the thread's line index are laundered through an
-// empty asm statement before every transform: otherwise the compiler hoists
-// all loop-invariant twiddle loads of the NIN + NOUT transforms to the kernel
-// entry and keeps them live, which costs over 60 VGPRs.
+// Spectra are in the column-tiled layout Z (fft_pass.h): a workgroup owns B
+// columns, whole tiles, so each of its loads and stores walks contiguous
+// memory.  The twiddle table is staged in LDS once per workgroup: the
+// per-stage twiddle reads of the 12 transforms of the elastic operation
+// would otherwise be a chain of dependent global loads.
 template <typename T, int L, class Op>
 __global__ void __launch_bounds__((TileZF<T, typename ZPlan<L, Op::PLAN>::type, Op::MAXT>::THREADS),
                                   (TileZF<T, typename ZPlan<L, Op::PLAN>::type, Op::MAXT>::template minw<Op::MINW>()))
@@ -70,8 +71,12 @@ k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
   using TL = TileZF<T, P, Op::MAXT>;
   if (gated_off(op.gate)) return;
   constexpr int TPL = P::TPL, E = P::E, B = TL::B;
+  static_assert(B % kTW == 0, "a workgroup owns whole tiles");
   __shared__ C2<T> lds[TL::LDS];
+  __shared__ C2<T> tw[L];
   const int t = threadIdx.x, b = t % B, tl = t / B, col = blockIdx.x * B + b, g = blockIdx.y;
+  for (int e = t; e < L; e += TL::THREADS) tw[e] = twL[e];
+  __syncthreads();
   const bool valid = col < ncols;
   const int colc = valid ? col : 0;
   const ColK ck = colk(op.P, colc);
@@ -80,9 +85,9 @@ k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
   // the column of input f+1 is loaded while input f is transformed
   C2<T> vn[E];
   {
-    const C2<T>* in = op.input(g, 0) + col;
+    const C2<T>* in = op.input(g, 0);
 #pragma unroll
-    for (int q = 0; q < E; ++q) vn[q] = valid ? in[(size_t)(tl + TPL * q) * ncols] : mk<T>(0, 0);
+    for (int q = 0; q < E; ++q) vn[q] = valid ? in[zcol(col, tl + TPL * q, L)] : mk<T>(0, 0);
   }
 #pragma unroll 1
   for (int f = 0; f < Op::NIN; ++f) {
@@ -90,31 +95,25 @@ k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = vn[q];
     if (f + 1 < Op::NIN) {
-      const C2<T>* in = op.input(g, f + 1) + col;
+      const C2<T>* in = op.input(g, f + 1);
 #pragma unroll
-      for (int q = 0; q < E; ++q) vn[q] = valid ? in[(size_t)(tl + TPL * q) * ncols] : mk<T>(0, 0);
+      for (int q = 0; q < E; ++q) vn[q] = valid ? in[zcol(col, tl + TPL * q, L)] : mk<T>(0, 0);
     }
     if (f > 0) __syncthreads();
-    const C2<T>* tw = twL;
-    int tlx = tl;
-    asm volatile("" : "+s"(tw), "+v"(tlx));
-    line_fft<T, -1, P>(v, tlx, lds, idx, tw);
-    op.template absorb<E>(st, g, f, v, ck, tlx, TPL);
+    line_fft<T, -1, P>(v, tl, lds, idx, tw);
+    op.template absorb<E>(st, g, f, v, ck, tl, TPL);
   }
   op.template finish<E>(st, g, ck, colc, tl, TPL, ncols, valid);
 #pragma unroll 1
   for (int o = 0; o < Op::NOUT; ++o) {
     C2<T> v[E];
-    const C2<T>* tw = twL;
-    int tlx = tl;
-    asm volatile("" : "+s"(tw), "+v"(tlx));
-    op.template emit<E>(st, g, o, v, ck, tlx, TPL);
+    op.template emit<E>(st, g, o, v, ck, tl, TPL);
     __syncthreads();
-    line_fft<T, +1, P>(v, tlx, lds, idx, tw);
-    C2<T>* out = op.output(g, o) + col;
+    line_fft<T, +1, P>(v, tl, lds, idx, tw);
+    C2<T>* out = op.output(g, o);
     if (valid) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) out[(size_t)(tl + TPL * q) * ncols] = v[q];
+      for (int q = 0; q < E; ++q) out[zcol(col, tl + TPL * q, L)] = v[q];
     }
   }
   op.template reduce<TL::NW>(st);
@@ -144,9 +143,9 @@ template <typename T> struct OpGrad {
   template <int E>
   __device__ void finish(State<E>& st, int g, const ColK&, int col, int tl, int TPL, int ncols, bool valid) const {
     if (!valid) return;
-    C2<T>* dst = sphi + (size_t)(v0 + g) * nc + col;
+    C2<T>* dst = sphi + (size_t)(v0 + g) * nc;
 #pragma unroll
-    for (int q = 0; q < E; ++q) dst[(size_t)(tl + TPL * q) * ncols] = st.s[q];
+    for (int q = 0; q < E; ++q) dst[zcol(col, tl + TPL * q, P->nz)] = st.s[q];
   }
   template <int O, int E>
   __device__ void emit_o(const State<E>& st, int g, C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
@@ -181,7 +180,7 @@ template <typename T> struct OpGrad {
 // R = kf . x_hat, u = Q^-1 R (u(0) = 0); Parseval sums of |u - u_old|^2 and
 // |u_old|^2 over the half spectrum; out: engineering-strain spectra (in place).
 template <typename T> struct OpElastic {
-  static constexpr int NIN = 6, NOUT = 6, MAXT = 512, MINW = 1;
+  static constexpr int NIN = 6, NOUT = 6, MAXT = 256, MINW = 2;
   static constexpr int PLAN = 1;
   const DevParams* P;
   C2<T>* work;
@@ -255,7 +254,7 @@ template <typename T> struct OpElastic {
       if (valid) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          C2<T>* uh = uhat + (size_t)i * nc + (size_t)kz * ncols + col;
+          C2<T>* uh = uhat + (size_t)i * nc + zcol(col, kz, P->nz);
           const C2<T> uo = *uh;
           const double dr = u.re[i] - (double)uo.x, di = u.im[i] - (double)uo.y;
           st.acc[2 * i] += w * (dr * dr + di * di);
@@ -308,7 +307,7 @@ template <typename T, int PLAN_, int MAXT_, int MINW_> struct OpElasticV : OpEla
   static constexpr int PLAN = PLAN_, MAXT = MAXT_, MINW = MINW_;
   OpElasticV(const OpElastic<T>& b) : OpElastic<T>(b) {}
 };
-constexpr int kZfElasticVariants = 6;
+constexpr int kZfElasticVariants = 2;
 
 // ------------------------------------------------------------ Cahn-Hilliard
 // in: work[4e] = c, work[4e+1..3] = Fx, Fy, Fz after x and y; out: c_hat new.
@@ -385,10 +384,10 @@ template <typename T> struct OpAC {
   }
   template <int E>
   __device__ void finish(State<E>& st, int g, const ColK&, int col, int tl, int TPL, int ncols, bool valid) const {
-    const C2<T>* src = sphi + (size_t)g * nc + col;
+    const C2<T>* src = sphi + (size_t)g * nc;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const C2<T> s = valid ? src[(size_t)(tl + TPL * q) * ncols] : mk<T>(0, 0);
+      const C2<T> s = valid ? src[zcol(col, tl + TPL * q, P->nz)] : mk<T>(0, 0);
       st.pr[q] = (double)s.x;
       st.pi[q] = (double)s.y;
     }

@@ -36,11 +36,7 @@ template <typename T>
 int launch_zf_elastic(const Fft3& f, const OpElastic<T>& op, int v, hipStream_t s, int probe_kind, double bytes,
                       int* nblk) {
   switch (v) {
-    case 1: return launch_zfused<T>(f, OpElasticV<T, 1, 512, 3>(op), 1, s, probe_kind, bytes, nblk);
-    case 2: return launch_zfused<T>(f, OpElasticV<T, 1, 512, 4>(op), 1, s, probe_kind, bytes, nblk);
-    case 3: return launch_zfused<T>(f, OpElasticV<T, 1, 256, 3>(op), 1, s, probe_kind, bytes, nblk);
-    case 4: return launch_zfused<T>(f, OpElasticV<T, 1, 256, 4>(op), 1, s, probe_kind, bytes, nblk);
-    case 5: return launch_zfused<T>(f, OpElasticV<T, 1, 1024, 2>(op), 1, s, probe_kind, bytes, nblk);
+    case 1: return launch_zfused<T>(f, OpElasticV<T, 1, 512, 1>(op), 1, s, probe_kind, bytes, nblk);
     default: return launch_zfused<T>(f, op, 1, s, probe_kind, bytes, nblk);
   }
 }
