@@ -52,11 +52,11 @@ def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind, graph=False
     sim.upload(phi=phi, cons=cons, u=np.zeros((3, *shape)), gradu=np.zeros((9, *shape)))
     if graph:
         sim.set_graph(True)
-    var = (ctypes.c_int32 * 4)()
-    tms = (ctypes.c_float * 12)()
+    var = (ctypes.c_int32 * 5)()
+    tms = (ctypes.c_float * 15)()
     _lib.check(lib.mm_get_tuning(sim.ctx, var, tms), "mm_get_tuning")
     tuning = {k: {"variant": var[i], "ms": [round(tms[3 * i + v], 4) for v in range(3)]}
-              for i, k in enumerate(["x_r2c", "x_c2r", "y", "z"])}
+              for i, k in enumerate(["x_r2c", "x_c2r", "y", "z", "y_tiled"])}
     sim.step(1 + warmup)  # elastep0 step + warm-up
     _lib.check(lib.mm_probe(sim.ctx, probe_kind), "mm_probe")
     torch.cuda.synchronize()

@@ -222,10 +222,12 @@ int mm_probe(mm_ctx *ctx, int32_t kind);
 int mm_probe_read(mm_ctx *ctx, double *ms_total, int64_t *launches, double *bytes_total);
 
 /* FFT plan autotuner (no reference counterpart): at mm_create / mm_fft_create
- * every radix plan of the x_r2c, x_c2r, y and z passes (default, lean,
- * 4-point) is timed on the grid's own size and the fastest kept; set the
- * environment variable MM_AUTOTUNE=0 to keep the defaults.  variant[4] gets
- * the chosen plan per pass, ms[12] the measured kernel times (ms, 4 fields). */
+ * every radix plan of the x_r2c, x_c2r, y and z passes and of the y pass of
+ * the fused path (forward + inverse with the change to the column-tiled
+ * layout) (default, lean, 4-point) is timed on the grid's own size and the
+ * fastest kept; set the environment variable MM_AUTOTUNE=0 to keep the
+ * defaults.  variant[5] gets the chosen plan per pass, ms[15] the measured
+ * kernel times (ms, 4 fields). */
 int mm_get_tuning(const mm_ctx *ctx, int32_t *variant, float *ms);
 
 /* ---- FFT interface: replaces f_trans / inv_trans over P3DFFT (ModFunc.f:15-40) ----

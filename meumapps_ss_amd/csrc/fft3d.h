@@ -28,10 +28,10 @@ struct Probe {
 struct Fft3 {
   int nx = 0, ny = 0, nz = 0, H = 0, prec = 0;
   Probe* probe = nullptr;
-  // plan variant per pass (x_r2c, x_c2r, y, z): 0 default, 1 lean, 2 lean-4;
-  // chosen by autotune() from the kernel times in tune_ms
-  int var[4] = {0, 0, 0, 0};
-  float tune_ms[4][3] = {};
+  // plan variant per pass (x_r2c, x_c2r, y, z, tiled y): 0 default, 1 lean,
+  // 2 lean-4; chosen by autotune() from the kernel times in tune_ms
+  int var[5] = {0, 0, 0, 0, 0};
+  float tune_ms[5][3] = {};
   // when non-null every pass launched through this plan is gated on *gate
   // (set while recording the graph-mode step)
   const int* gate = nullptr;

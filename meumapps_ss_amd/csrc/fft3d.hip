@@ -256,7 +256,7 @@ int Fft3::y_pass_t(void* const* in, void* const* out, int nf, int dir, int mode,
   Batch<T> b = make_batch<T>((const void* const*)in, out, nf, gate);
   if (probe) probe->begin(MM_PROBE_Y, s);
   switch (ny) {
-#define X(L_) case L_: MM_VARIANTS(go_yt, L_, var[2], b, nf, twNy, nz, H, dir, mode, s); break;
+#define X(L_) case L_: MM_VARIANTS(go_yt, L_, var[4], b, nf, twNy, nz, H, dir, mode, s); break;
     MM_NLENS(X)
 #undef X
     default: return MM_ERR_ARG;
@@ -338,26 +338,28 @@ static int autotune_t(Fft3& f, hipStream_t s) {
   void* re = nullptr;
   void* sp = nullptr;
   if (hipMalloc(&re, nf * f.nreal() * sizeof(T)) != hipSuccess) return MM_ERR_NOMEM;
-  if (hipMalloc(&sp, nf * f.ncplx() * 2 * sizeof(T)) != hipSuccess) { (void)hipFree(re); return MM_ERR_NOMEM; }
+  if (hipMalloc(&sp, 2 * nf * f.ncplx() * 2 * sizeof(T)) != hipSuccess) { (void)hipFree(re); return MM_ERR_NOMEM; }
   MM_HIP(hipMemsetAsync(re, 0, nf * f.nreal() * sizeof(T), s));
-  MM_HIP(hipMemsetAsync(sp, 0, nf * f.ncplx() * 2 * sizeof(T), s));
+  MM_HIP(hipMemsetAsync(sp, 0, 2 * nf * f.ncplx() * 2 * sizeof(T), s));
   const T* rin[4];
   T* rout[4];
   void* sps[4];
+  void* sps2[4];
   for (int i = 0; i < nf; ++i) {
     rin[i] = (const T*)re + i * f.nreal();
     rout[i] = (T*)re + i * f.nreal();
     sps[i] = (C2<T>*)sp + i * f.ncplx();
+    sps2[i] = (C2<T>*)sp + (nf + i) * f.ncplx();
   }
   hipEvent_t a, b;
   MM_HIP(hipEventCreate(&a));
   MM_HIP(hipEventCreate(&b));
   Probe* keep = f.probe;
   f.probe = nullptr;
-  const int lens[4] = {f.nx / 2, f.nx / 2, f.ny, f.nz};
+  const int lens[5] = {f.nx / 2, f.nx / 2, f.ny, f.nz, f.ny};
   int rc = 0;
-  for (int k = 0; k < 4 && !rc; ++k) {
-    if (lens[k] < 64) { f.var[k] = 0; continue; }
+  for (int k = 0; k < 5 && !rc; ++k) {
+    if (lens[k] < 64 || (k == 4 && f.ny % kTW)) { f.var[k] = 0; continue; }
     float best = 1e30f;
     int bestv = 0;
     for (int v = 0; v < 3 && !rc; ++v) {
@@ -368,7 +370,12 @@ static int autotune_t(Fft3& f, hipStream_t s) {
         if (k == 0) rc = f.x_r2c<T>(rin, sps, nf, s, (T)1);
         else if (k == 1) rc = f.x_c2r<T>(sps, rout, nf, s);
         else if (k == 2) rc = f.y_pass<T>(sps, sps, nf, -1, s);
-        else rc = f.z_pass<T>(sps, sps, nf, -1, s);
+        else if (k == 3) rc = f.z_pass<T>(sps, sps, nf, -1, s);
+        else {
+          // the tiled y pass of the fused path: forward S -> Z and inverse Z -> S
+          rc = f.y_pass_t<T>(sps, sps2, nf, -1, 1, s);
+          if (!rc) rc = f.y_pass_t<T>(sps2, sps, nf, +1, 2, s);
+        }
         MM_HIP(hipEventRecord(b, s));
         MM_HIP(hipEventSynchronize(b));
         float t = 0.f;

@@ -1900,7 +1900,7 @@ extern "C" int mm_probe_read(mm_ctx* c, double* ms, int64_t* launches, double* b
 
 extern "C" int mm_get_tuning(const mm_ctx* c, int32_t* variant, float* ms) {
   if (!c) return MM_ERR_ARG;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     if (variant) variant[k] = c->fft.var[k];
     for (int v = 0; v < 3; ++v)
       if (ms) ms[k * 3 + v] = c->fft.tune_ms[k][v];
