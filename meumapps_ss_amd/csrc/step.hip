@@ -431,8 +431,13 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, C2<T>* wo
           const double* __restrict__ cfac_dev, const int* __restrict__ gate) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
   __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  // twiddle tables staged in LDS: the 12 transforms read them at every stage
+  __shared__ C2<T> twl[L], twn[2 * L];
   if (gated_off(gate)) return;
   const int z = blockIdx.y, y0 = blockIdx.x * B, t = threadIdx.x;
+  for (int e = t; e < L; e += TileLM<T, P>::THREADS) twl[e] = twL[e];
+  for (int e = t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
+  __syncthreads();
   // line view (transforms, pointwise) and gather view (transposed loads and stores)
   const int bl = t / TPL, tl = t % TPL, yl = y0 + bl;
   const int bg = t % B, r = t / B, yg = y0 + bg;
@@ -463,13 +468,13 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, C2<T>* wo
       if (k == 0) { xk.y = 0; xc.y = 0; }
       C2<T> ev = mk<T>(xk.x + xc.x, xk.y - xc.y);
       C2<T> dd = mk<T>(xk.x - xc.x, xk.y + xc.y);
-      C2<T> w = twN[k];
+      C2<T> w = twn[k];
       w.y = -w.y;
       C2<T> od = cmul<T>(dd, w);
       v[q] = mk<T>(ev.x - od.y, ev.y + od.x);
     }
     __syncthreads();
-    line_fft<T, +1, P>(v, tl, lds, idx, twL);
+    line_fft<T, +1, P>(v, tl, lds, idx, twl);
 #pragma unroll
     for (int q = 0; q < E; ++q) s[m][q] = v[q];
   }
@@ -508,7 +513,7 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, C2<T>* wo
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = s[m][q];
     __syncthreads();
-    line_fft<T, -1, P>(v, tl, lds, idx, twL);
+    line_fft<T, -1, P>(v, tl, lds, idx, twl);
     if constexpr (P::NS > 1) __syncthreads();
 #pragma unroll
     for (int q = 0; q < E; ++q) lds[idx(tl + TPL * q)] = v[q];
@@ -524,7 +529,7 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, C2<T>* wo
         C2<T> zc = lds[bg * LP + k2 + (k2 >> 4)];
         C2<T> ev = mk<T>(zk.x + zc.x, zk.y - zc.y);
         C2<T> od = mk<T>(zk.y + zc.y, -(zk.x - zc.x));
-        C2<T> w = twN[kx];
+        C2<T> w = twn[kx];
         C2<T> xx = cadd<T>(ev, cmul<T>(w, od));
         out[((size_t)z * (L + 1) + kx) * ny + yg] = cscale<T>(xx, hs);
       }
