@@ -210,8 +210,11 @@ __global__ void __launch_bounds__((TileLM<T, P>::THREADS))
 k_y_pass_t(Batch<T> bt, const C2<T>* __restrict__ twL, int nz, int H) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
   __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  __shared__ C2<T> tw[L];
   if (gated_off(bt.gate)) return;
   const int f = blockIdx.z, t = threadIdx.x;
+  for (int e = t; e < L; e += TileLM<T, P>::THREADS) tw[e] = twL[e];
+  __syncthreads();
   const int b = t / TPL, tl = t % TPL, line = blockIdx.x * B + b;
   const bool ok = line < nz * H;
   const int kx = ok ? line / nz : 0, kz = ok ? line % nz : 0;
@@ -230,7 +233,7 @@ k_y_pass_t(Batch<T> bt, const C2<T>* __restrict__ twL, int nz, int H) {
     for (int q = 0; q < E; ++q) v[q] = mk<T>(0, 0);
   }
   IdxLineMajor idx{b * LP};
-  line_fft<T, DIR, P>(v, tl, lds, idx, twL);
+  line_fft<T, DIR, P>(v, tl, lds, idx, tw);
   if (ok) {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
