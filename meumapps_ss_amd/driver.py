@@ -1,0 +1,81 @@
+"""Host driver of a run: the reference main program around the C ABI.
+
+Mirrors Multi_MEUMAPPS.f for a restart run (nrun = 2): read Input.txt
+(:117-206) and set up (:218-448, params.setup), read the restart record set
+data_save.0001 (:466-481), write the initial dump step_{step-1} (:496-520),
+then advance the loop body (:548-1106) through mm_step in chunks that end at
+the dump steps, writing the fort.98 / fort.99 rows of every step and the dump
+step_NNNNNNN/{data_extract_mult.in, data_save.0001} whenever the step counter
+reaches a multiple of ifreq (:1069-1100).  The loop runs the body for
+step = step0 .. Nt - 1, as the reference's ``if (step .lt. Nt) go to 4``.
+
+Initial conditions other than a restart file (InitFields / NucAdd,
+ModInitials.f:415-826) and Langevin noise are host work outside this path:
+run() refuses nrun != 2 and noise_step >= step.
+
+    python -m meumapps_ss_amd.driver Input.txt data_save.0001 outdir [--mixed]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+from typing import Optional
+
+from .params import MM_FP64, MM_MIXED, read_input_txt, setup
+from .records import dump_step, fort98_row, fort99_row, read_data_save
+from .sim import Simulation
+
+
+def run(input_txt: str, restart: str, outdir: str, precision: int = MM_FP64, sym_var: Optional[str] = None,
+        graph: bool = False) -> dict:
+    inp = read_input_txt(input_txt, sym_var)
+    if inp.nrun != 2:
+        raise NotImplementedError("only restart runs (nrun = 2): initial fields come from data_save.0001")
+    p = setup(inp)
+    if p.noise_step >= p.step:
+        raise NotImplementedError("Langevin noise steps (step <= noise_step) are not run on the GPU")
+    shape = (p.nx, p.ny, p.nz)
+    V, E = p.vartot, p.numel
+    rec = read_data_save(restart, V, E)
+    if tuple(rec["shape"]) != shape:
+        raise ValueError(f"restart grid {rec['shape']} differs from Input.txt {shape}")
+    os.makedirs(outdir, exist_ok=True)
+    sim = Simulation(p, precision)
+    try:
+        sim.upload_records(rec)
+        if graph:
+            sim.set_graph(True)
+        step = p.step
+        dump_step(outdir, step - 1, shape, V, E, sim.download_records())
+        dumps = [step - 1]
+        with open(os.path.join(outdir, "fort.98"), "w") as f98, open(os.path.join(outdir, "fort.99"), "w") as f99:
+            while step < inp.nt:
+                nxt = (step // inp.ifreq + 1) * inp.ifreq
+                chunk = min(nxt, inp.nt) - step
+                rows = sim.step(chunk)
+                for r in rows:
+                    f98.write(fort98_row(r) + "\n")
+                    f99.write(fort99_row(r, E) + "\n")
+                step += chunk
+                if step % inp.ifreq == 0:
+                    dump_step(outdir, step, shape, V, E, sim.download_records())
+                    dumps.append(step)
+    finally:
+        sim.close()
+    return {"steps": inp.nt - p.step, "dumps": dumps, "final_step": step}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("input_txt")
+    ap.add_argument("restart")
+    ap.add_argument("outdir")
+    ap.add_argument("--mixed", action="store_true")
+    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--sym-var", default=None)
+    a = ap.parse_args()
+    print(run(a.input_txt, a.restart, a.outdir, MM_MIXED if a.mixed else MM_FP64, a.sym_var, a.graph))
+
+
+if __name__ == "__main__":
+    main()

@@ -88,6 +88,22 @@ class Simulation:
         return dict(phi=aos_to_soa(phi, self.V, self.shape), cons=aos_to_soa(cons, self.E, self.shape),
                     el_en=el, u=u, gradu=gu)
 
+    def download_records(self) -> dict:
+        """The arrays of the reference's data_save records, as mm_download
+        returns them: phi / cons component-fastest (flat), el_en, u (3, N^3)
+        = u1..u3, gradu (9, N^3) in record order (grad_1_u1, grad_1_u2, ...)."""
+        n3 = int(np.prod(self.shape))
+        phi = np.empty(self.V * n3); cons = np.empty(self.E * n3)
+        el = np.empty(n3); u = np.empty(3 * n3); gu = np.empty(9 * n3)
+        _lib.check(self.lib.mm_download(self.ctx, _ptr(phi), _ptr(cons), _ptr(el), _ptr(u), _ptr(gu)),
+                   "mm_download")
+        return dict(phi=phi, cons=cons, el_en=el, u=u.reshape(3, n3), gradu=gu.reshape(9, n3))
+
+    def upload_records(self, rec: dict):
+        """Inverse of download_records (the restart read of Multi_MEUMAPPS.f:470-481)."""
+        arrs = [np.ascontiguousarray(rec[k], dtype=np.float64).ravel() for k in ("phi", "cons", "el_en", "u", "gradu")]
+        _lib.check(self.lib.mm_upload(self.ctx, *[_ptr(a) for a in arrs]), "mm_upload")
+
     def field(self, name: str) -> np.ndarray:
         which = FIELD[name]
         ptr = ctypes.c_void_p(); esz = ctypes.c_int32(); nf = ctypes.c_int32()
@@ -168,6 +184,6 @@ class Ensemble:
 
 def fort98_line(d: dict) -> str:
     """One fort.98 row, format (i8,10e15.7) (Multi_MEUMAPPS.f:905-906,1113)."""
-    vals = [d["fint"], d["fw"], d["fch"], d["elen"], d["etot"], d["glob_min"], d["glob_max"],
-            d["phi_max"], d["solid_tot"]]
-    return f"{d['step']:8d}" + "".join(f"{v:15.7E}" for v in vals)
+    from .records import fort98_row
+
+    return fort98_row(d)
