@@ -414,7 +414,12 @@ extern "C" int mm_fft_create(int32_t nx, int32_t ny, int32_t nz, int32_t precisi
   mm_fft* h = new mm_fft();
   int rc = h->f.init(nx, ny, nz, precision);
   const char* at = std::getenv("MM_AUTOTUNE");
-  if (!rc && !(at && at[0] == '0')) rc = h->f.autotune(nullptr);
+  const char* fv = std::getenv("MM_FFT_VARIANT");  // 0, 1, 2: every pass on that plan, no timing
+  if (!rc && fv && fv[0] >= '0' && fv[0] <= '2') {
+    for (int k = 0; k < 5; ++k) h->f.var[k] = fv[0] - '0';
+  } else if (!rc && !(at && at[0] == '0')) {
+    rc = h->f.autotune(nullptr);
+  }
   if (rc) {
     h->f.destroy();
     delete h;

@@ -184,7 +184,7 @@ template <> struct PlanLean4Sel<32> { using type = PlanT<32, 4, 4, 4, 2, 1>; };
 template <> struct PlanLean4Sel<64> { using type = PlanT<64, 4, 4, 4, 4, 1>; };
 template <> struct PlanLean4Sel<128> { using type = PlanT<128, 4, 4, 4, 4, 2>; };
 template <> struct PlanLean4Sel<256> { using type = PlanT<256, 4, 4, 4, 4, 4>; };
-template <> struct PlanLean4Sel<512> { using type = PlanT<512, 8, 8, 8, 8, 1>; };
+template <> struct PlanLean4Sel<512> { using type = PlanT<512, 4, 4, 4, 4, 4, 2>; };
 template <int L> using PlanLean4 = typename PlanLean4Sel<L>::type;
 
 template <class P, int S> struct StageR {

@@ -22,7 +22,9 @@ namespace mm {
 // thread (radix-3 lengths) use 256 so that their registers fit.
 template <typename T, class P, int MAXT> struct TileZF {
   static constexpr int TPL = P::TPL;
-  static constexpr int MT = P::E > 8 ? cmin(MAXT, 256) : MAXT;
+  // at least kTW columns, so that a workgroup owns a whole tile of the layout
+  static constexpr int MT0 = P::E > 8 ? cmin(MAXT, 256) : MAXT;
+  static constexpr int MT = MT0 > kTW * TPL ? MT0 : kTW * TPL;
   // minimum waves per SIMD requested from the register allocator
   template <int MINW> static constexpr int minw() { return P::E <= 4 ? MINW : (MINW > 2 ? 2 : MINW); }
   static constexpr int B = pow2floor(cmin(MT / TPL, kLdsBudget / (P::L * (int)sizeof(C2<T>))));

@@ -34,6 +34,8 @@ CASES = {
                    {"cela2": {(1, 1): 300.0, (1, 2): 140.0, (4, 4): 120.0}}, "rich"),
     "v2_nonsq": (24, 2, (0.015, 0.015, -0.01), 0.005, {}, "rich"),
     "v1_e2": (16, 1, (0.01, 0.01, 0.01), 0.005, {"numel_in": 3}, "rich"),
+    # 16 x 16 x 512: the length-512 z kernels (radix-4 plan of the fused elastic pass)
+    "v3_z512": (16, 3, (0.02, -0.01, -0.01), 0.005, {"nz": 512}, "rich"),
 }
 
 
@@ -41,7 +43,9 @@ def make_case(name, seed=0):
     n, nvar, tstr, dt, kw, init = CASES[name]
     kw = dict(kw)
     numel_in = kw.pop("numel_in", 2)
+    nz = kw.pop("nz", n)
     inp = model_inputs(n, nvar, tstr, dt, **kw)
+    inp.nz = nz
     if numel_in != 2:
         # a second composition: A, G0, c0 rows extend to 2*numel+1
         inp.numel_in = numel_in
@@ -51,7 +55,7 @@ def make_case(name, seed=0):
         inp.conc = [(1, 0.4, 1.0, 0.0), (2, 0.25, 1.0, 0.0)]
     p = setup(inp)
     V, E = p.vartot, p.numel
-    shape = (n, n, n)
+    shape = (nz, n, n)
     rng = np.random.default_rng(seed + 17 * n + 5 * V)
     if init == "noise":
         phi = rng.uniform(-0.01, 0.01, (V, *shape))

@@ -35,15 +35,25 @@ def _numpy_to_spec(a):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("variant", [None, "0", "1", "2"])
 @pytest.mark.parametrize("shape", SIZES)
-def test_forward_inverse(gpu_lib, shape, prec):
+def test_forward_inverse(gpu_lib, shape, prec, variant):
+    """variant None: the autotuner's plans; 0/1/2: every pass forced to the
+    default, lean (radix 8) or lean-4 (radix 4) plan of its length."""
+    import os
+
     import torch
 
     nx, ny, nz = shape
     rng = np.random.default_rng(nx * 1000 + ny * 10 + nz)
     x = rng.standard_normal((nz, ny, nx))
     h = ctypes.c_void_p()
-    _lib.check(gpu_lib.mm_fft_create(nx, ny, nz, prec, ctypes.byref(h)), "mm_fft_create")
+    if variant is not None:
+        os.environ["MM_FFT_VARIANT"] = variant
+    try:
+        _lib.check(gpu_lib.mm_fft_create(nx, ny, nz, prec, ctypes.byref(h)), "mm_fft_create")
+    finally:
+        os.environ.pop("MM_FFT_VARIANT", None)
     try:
         H = nx // 2 + 1
         nc = gpu_lib.mm_fft_spectrum_elems(h)
