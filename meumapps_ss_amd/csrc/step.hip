@@ -24,19 +24,33 @@ int launch_zf_elastic(const Fft3& f, const OpElastic<T>& op, int v, hipStream_t 
 
 // ============================================================== kernels
 
-__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ part, int nblk, int K,
-                                                     FinalizeOps ops, double* __restrict__ out,
-                                                     const int* __restrict__ gate) {
+// One workgroup of kFinBlock threads; each thread keeps four independent
+// partial results (fixed assignment of partials to accumulators), so the
+// loads of a slot are in flight together; the combination order is fixed.
+constexpr int kFinBlock = 1024;
+template <int OP> __device__ __forceinline__ double fin_slot(const double* __restrict__ p, int nblk, double* sh) {
+  const double id = OP == 0 ? 0.0 : (OP == 1 ? HUGE_VAL : -HUGE_VAL);
+  double a0 = id, a1 = id, a2 = id, a3 = id;
+  int b = threadIdx.x;
+  for (; b + 3 * kFinBlock < nblk; b += 4 * kFinBlock) {
+    const double x0 = p[b], x1 = p[b + kFinBlock], x2 = p[b + 2 * kFinBlock], x3 = p[b + 3 * kFinBlock];
+    a0 = rop<OP>(a0, x0);
+    a1 = rop<OP>(a1, x1);
+    a2 = rop<OP>(a2, x2);
+    a3 = rop<OP>(a3, x3);
+  }
+  for (; b < nblk; b += kFinBlock) a0 = rop<OP>(a0, p[b]);
+  return block_reduce<OP, kFinBlock / 64>(rop<OP>(rop<OP>(a0, a1), rop<OP>(a2, a3)), sh);
+}
+__global__ void __launch_bounds__(kFinBlock) k_finalize(const double* __restrict__ part, int nblk, int K,
+                                                        FinalizeOps ops, double* __restrict__ out,
+                                                        const int* __restrict__ gate) {
   if (gated_off(gate)) return;
-  __shared__ double sh[kWaves];
+  __shared__ double sh[kFinBlock / 64];
   for (int k = 0; k < K; ++k) {
+    const double* p = part + (size_t)k * nblk;
     const int op = ops.op[k];
-    double acc = op == 0 ? 0.0 : (op == 1 ? HUGE_VAL : -HUGE_VAL);
-    for (int b = threadIdx.x; b < nblk; b += kBlock) {
-      const double x = part[(size_t)k * nblk + b];
-      acc = op == 0 ? acc + x : (op == 1 ? (x < acc ? x : acc) : (x > acc ? x : acc));
-    }
-    double r = op == 0 ? block_reduce<0>(acc, sh) : (op == 1 ? block_reduce<1>(acc, sh) : block_reduce<2>(acc, sh));
+    const double r = op == 0 ? fin_slot<0>(p, nblk, sh) : (op == 1 ? fin_slot<1>(p, nblk, sh) : fin_slot<2>(p, nblk, sh));
     if (threadIdx.x == 0) out[k] = r;
   }
 }
@@ -1087,7 +1101,7 @@ int finalize(mm_ctx* c, int nblk, int K, std::initializer_list<int> ops, int slo
   int i = 0;
   for (int o : ops) fo.op[i++] = o;
   for (; i < 8; ++i) fo.op[i] = 0;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->s, c->part, nblk, K, fo, c->dres + slot0, c->gate);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, c->s, c->part, nblk, K, fo, c->dres + slot0, c->gate);
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -1419,7 +1433,7 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   for (int e = 0; e < E; ++e) fo.op[e] = 0;
   fo.op[E] = 1; fo.op[E + 1] = 2; fo.op[E + 2] = 2; fo.op[E + 3] = 0;
   for (int i = E + 4; i < 8; ++i) fo.op[i] = 0;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->s, c->part, c->nblk_pw, E + 4, fo, c->dres + R_CLAMP,
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, c->s, c->part, c->nblk_pw, E + 4, fo, c->dres + R_CLAMP,
                      c->gate);
   MM_HIP(hipGetLastError());
   return 0;

@@ -138,12 +138,11 @@ __device__ __forceinline__ void block_store(double v, double* part, int k, doubl
   if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = r;
 }
 
-// one block: out[k] = reduce_op[k] over part[k*nblk + 0..nblk)
+// reduction op per slot of k_finalize (out[k] = reduce_op[k] over
+// part[k*nblk + 0..nblk)): 0 sum, 1 min, 2 max
 struct FinalizeOps {
   int op[8];
 };
-__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ part, int nblk, int K,
-                                                     FinalizeOps ops, double* __restrict__ out);
 
 // ---------------------------------------------------------- k-space helpers
 // kf = i*a with a = m*fl(2 pi/N), m wrapped to (-N/2, N/2]; kf zeroed at the
