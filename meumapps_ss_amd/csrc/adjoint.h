@@ -718,7 +718,10 @@ extern "C" int mm_adj_enable(mm_ctx* c) {
   if (rc) { adj_release(a); return rc; }
   c->adj = a;
   // the fused x path computes the same iteration with other FFT plans; the
-  // tape replays the step bitwise only through one code path
+  // tape replays the step bitwise only through one code path (the real
+  // strain is formed from its x half-spectra first)
+  rc = strain_to_real_any(c);
+  if (rc) { adj_release(a); return rc; }
   c->xpolar = false;
   if (c->gx) { (void)hipGraphExecDestroy(c->gx); c->gx = nullptr; }
   if (c->g) { (void)hipGraphDestroy(c->g); c->g = nullptr; }
@@ -776,6 +779,7 @@ extern "C" int mm_state_save(mm_ctx* c, void* dst) {
   p += sizeof(h);
   MM_TRY(d2d(c, p, c->phi, c->V * n3 * es)); p += c->V * n3 * es;
   MM_TRY(d2d(c, p, c->cons, c->E * n3 * es)); p += c->E * n3 * es;
+  MM_TRY(strain_to_real_any(c));
   MM_TRY(d2d(c, p, c->strain, 6 * n3 * es)); p += 6 * n3 * es;
   MM_TRY(d2d(c, p, c->suhat, 3 * c->nc * 2 * es));
   MM_HIP(hipStreamSynchronize(c->s));
@@ -793,6 +797,7 @@ extern "C" int mm_state_load(mm_ctx* c, const void* src) {
   MM_TRY(d2d(c, c->phi, p, c->V * n3 * es)); p += c->V * n3 * es;
   MM_TRY(d2d(c, c->cons, p, c->E * n3 * es)); p += c->E * n3 * es;
   MM_TRY(d2d(c, c->strain, p, 6 * n3 * es)); p += 6 * n3 * es;
+  MM_TRY(strain_from_real_any(c));
   MM_TRY(d2d(c, c->suhat, p, 3 * c->nc * 2 * es));
   MM_HIP(hipStreamSynchronize(c->s));
   c->step = (int)h[0];

@@ -428,6 +428,64 @@ k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __r
   if (bad) *err = MM_ERR_CINV_ITER;
 }
 
+// ------------------------------------------- fused x-pass kernels: shared parts
+// One workgroup owns B rows (z, y0 .. y0+B-1) of x half-spectra; each thread
+// keeps the points x = 2n, 2n+1 (n = tl + TPL q) of its row through every
+// phase, so pointwise work between the x transforms needs no exchange.
+template <typename T, class P> struct XRows {
+  static constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  int z, y0, t, bl, tl, yl, bg, r, yg;
+  __device__ XRows(int ny) {
+    z = blockIdx.y; y0 = blockIdx.x * B; t = threadIdx.x;
+    bl = t / TPL; tl = t % TPL; yl = y0 + bl;  // line view (transforms, pointwise)
+    bg = t % B; r = t / B; yg = y0 + bg;       // gather view (transposed loads and stores)
+    (void)ny;
+  }
+};
+
+// c2r of six x half-spectra (in + m*nc, layout S after the inverse y pass) into
+// s[m][q] = (x[2n], x[2n+1]); imaginary parts of X_0 and X_L are ignored
+template <typename T, class P>
+__device__ __forceinline__ void x_c2r_six(const XRows<T, P>& w, const C2<T>* in, size_t nc, int ny, C2<T>* lds,
+                                          const C2<T>* twl, const C2<T>* twn, C2<T> (&s)[6][P::E]) {
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
+  const IdxLineMajor idx{w.bl * LP};
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const C2<T>* src = in + (size_t)m * nc;
+    C2<T> x[E + 1];
+#pragma unroll
+    for (int i = 0; i <= E; ++i) {
+      const int kx = w.r + TPL * i;
+      x[i] = (w.yg < ny && kx <= L) ? src[((size_t)w.z * (L + 1) + kx) * ny + w.yg] : mk<T>(0, 0);
+    }
+    if (m > 0) __syncthreads();
+#pragma unroll
+    for (int i = 0; i <= E; ++i) {
+      const int kx = w.r + TPL * i;
+      if (kx <= L) lds[w.bg * LP + kx + (kx >> 4)] = x[i];
+    }
+    __syncthreads();
+    C2<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int k = w.tl + TPL * q;
+      C2<T> xk = lds[idx(k)], xc = lds[idx(L - k)];
+      if (k == 0) { xk.y = 0; xc.y = 0; }
+      C2<T> ev = mk<T>(xk.x + xc.x, xk.y - xc.y);
+      C2<T> dd = mk<T>(xk.x - xc.x, xk.y + xc.y);
+      C2<T> tw = twn[k];
+      tw.y = -tw.y;
+      C2<T> od = cmul<T>(dd, tw);
+      v[q] = mk<T>(ev.x - od.y, ev.y + od.x);
+    }
+    __syncthreads();
+    line_fft<T, +1, P>(v, w.tl, lds, idx, twl);
+#pragma unroll
+    for (int q = 0; q < E; ++q) s[m][q] = v[q];
+  }
+}
+
 // ------------------------------------ fused x pass with the polarization stress
 // The elastic iterations after the first (ModElastic.f:241-323 between the
 // inverse transforms of the strain and the forward transforms of x): one
@@ -435,63 +493,27 @@ k_polar(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __r
 // the inverse y pass, runs their x c2r transforms into registers, forms the
 // polarization stress at every point of its rows (homogeneous moduli), runs
 // the six x r2c transforms and writes the spectra back in place.  The real
-// strain and stress never reach HBM.  Each thread keeps the same points
-// (x = 2n, 2n+1 for n = tl + TPL q) through all three phases, so the
-// pointwise phase needs no exchange.
+// strain and stress never reach HBM.  in and work may be the same buffer
+// (each workgroup reads its rows of every component before it writes).
 template <typename T, class P, int V>
 __global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
-k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, C2<T>* work, size_t nc, size_t n3,
-          const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T scale, double cfac_arg,
+k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<T>* in, C2<T>* work, size_t nc,
+          size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T scale, double cfac_arg,
           const double* __restrict__ cfac_dev, const int* __restrict__ gate) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
   __shared__ C2<T> lds[TileLM<T, P>::LDS];
   // twiddle tables staged in LDS: the 12 transforms read them at every stage
   __shared__ C2<T> twl[L], twn[2 * L];
   if (gated_off(gate)) return;
-  const int z = blockIdx.y, y0 = blockIdx.x * B, t = threadIdx.x;
+  const XRows<T, P> w(ny);
+  const int z = w.z, t = w.t, tl = w.tl, yl = w.yl, bg = w.bg, r = w.r, yg = w.yg;
   for (int e = t; e < L; e += TileLM<T, P>::THREADS) twl[e] = twL[e];
   for (int e = t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
   __syncthreads();
-  // line view (transforms, pointwise) and gather view (transposed loads and stores)
-  const int bl = t / TPL, tl = t % TPL, yl = y0 + bl;
-  const int bg = t % B, r = t / B, yg = y0 + bg;
-  const IdxLineMajor idx{bl * LP};
+  const IdxLineMajor idx{w.bl * LP};
   C2<T> s[6][E];
-  // 1. c2r of the six strain components (imaginary parts of X_0, X_L ignored)
-#pragma unroll
-  for (int m = 0; m < 6; ++m) {
-    const C2<T>* in = work + (size_t)m * nc;
-    C2<T> x[E + 1];
-#pragma unroll
-    for (int i = 0; i <= E; ++i) {
-      const int kx = r + TPL * i;
-      x[i] = (yg < ny && kx <= L) ? in[((size_t)z * (L + 1) + kx) * ny + yg] : mk<T>(0, 0);
-    }
-    if (m > 0) __syncthreads();
-#pragma unroll
-    for (int i = 0; i <= E; ++i) {
-      const int kx = r + TPL * i;
-      if (kx <= L) lds[bg * LP + kx + (kx >> 4)] = x[i];
-    }
-    __syncthreads();
-    C2<T> v[E];
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int k = tl + TPL * q;
-      C2<T> xk = lds[idx(k)], xc = lds[idx(L - k)];
-      if (k == 0) { xk.y = 0; xc.y = 0; }
-      C2<T> ev = mk<T>(xk.x + xc.x, xk.y - xc.y);
-      C2<T> dd = mk<T>(xk.x - xc.x, xk.y + xc.y);
-      C2<T> w = twn[k];
-      w.y = -w.y;
-      C2<T> od = cmul<T>(dd, w);
-      v[q] = mk<T>(ev.x - od.y, ev.y + od.x);
-    }
-    __syncthreads();
-    line_fft<T, +1, P>(v, tl, lds, idx, twl);
-#pragma unroll
-    for (int q = 0; q < E; ++q) s[m][q] = v[q];
-  }
+  // 1. c2r of the six strain components (in; may be the same buffer as work)
+  x_c2r_six<T, P>(w, in, nc, ny, lds, twl, twn, s);
   // 2. polarization stress at the 2E points of this thread
   if (yl < ny) {
     const double cfac = cfac_dev ? *cfac_dev : cfac_arg;
@@ -551,6 +573,123 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, C2<T>* wo
   }
 }
 
+// Elastic energy density and driving force at one point when every phase has
+// the same moduli (ModElastic.f:493-691): sig = C eps with
+// eps = strain + w e_mean - w sum_v p_v eps0_v, every phase's energy density
+// equals the matrix one, and el_grad_v = p'(phi_v)(el_mat - el_ppt) -
+// p'(phi_v) sig:eps0_v where sum_m trans_ppt(m,v) > 1e-9.  Returns el_en.
+template <int V, int NPH>
+__device__ __forceinline__ double elener_homog(const DevParams* __restrict__ P, const double (&f)[V],
+                                               const double (&strain)[6], double (&eg)[V]) {
+  double Pv[V], ps = 0.0, te[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    eg[v] = 0.0;
+    Pv[v] = pfun(f[v]);
+    ps = ps + Pv[v];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) te[m] = te[m] + Pv[v] * P->tp[v * 6 + m];
+  }
+  double eps[6], sig[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const double w = m < 3 ? 1.0 : 2.0;
+    eps[m] = strain[m] + w * P->e_mean[m] - w * te[m];
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s += P->cela[c * 6 + a] * eps[c];
+    sig[a] = s;
+  }
+  double e2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) e2 += sig[a] * eps[a];
+  const double el_mat = 0.5 * e2, el_ppt = el_mat;
+  double een = (1.0 - ps) * el_mat;
+#pragma unroll
+  for (int q = 2; q <= NPH; ++q) {
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (P->phase_of[v] == q) {
+        eg[v] = 0.0;
+        if (P->tsum_ok[v]) {
+          een = een + el_ppt * Pv[v];
+          const double* t = P->tp + v * 6;
+          const double d = (sig[0] * t[0] + sig[1] * t[1] + sig[2] * t[2] +
+                            2.0 * (sig[3] * t[3] + sig[4] * t[4] + sig[5] * t[5]));
+          const double pp = ppr(f[v]);
+          eg[v] = pp * (el_mat - el_ppt) - d * pp;
+        }
+      }
+  }
+  return een;
+}
+
+// ------------------------------------------ fused x c2r of the strain + elener
+// The strain is kept between steps as x half-spectra (the layout S after the
+// inverse y pass); this kernel runs their x c2r transforms into registers and
+// evaluates elener (homogeneous moduli) at every point of its rows: el_en,
+// el_grad, f_diff += el_grad, and the partial sums of el_en per workgroup
+// (part[blockIdx.y * gridDim.x + blockIdx.x]).  The real strain is not written.
+template <typename T, class P, int V, int NPH>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
+k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<T>* __restrict__ sstrain, size_t nc,
+           size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T* __restrict__ el_en,
+           T* __restrict__ elgrad, T* __restrict__ fdiff, double* __restrict__ part, const int* __restrict__ gate) {
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, NW = (TileLM<T, P>::THREADS + 63) / 64;
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  __shared__ C2<T> twl[L], twn[2 * L];
+  __shared__ double sh[NW];
+  if (gated_off(gate)) return;
+  const XRows<T, P> w(ny);
+  for (int e = w.t; e < L; e += TileLM<T, P>::THREADS) twl[e] = twL[e];
+  for (int e = w.t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
+  __syncthreads();
+  C2<T> s[6][E];
+  x_c2r_six<T, P>(w, sstrain, nc, ny, lds, twl, twn, s);
+  double s_el = 0.0;
+  if (w.yl < ny) {
+    const size_t row = ((size_t)w.z * ny + w.yl) * (size_t)(2 * L);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int n = w.tl + TPL * q;
+      C2<T> ph[V], fd[V], eo, go[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        ph[v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[n];
+        fd[v] = ((const C2<T>*)(fdiff + (size_t)v * n3 + row))[n];
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double f[V], e[6], eg[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) f[v] = (double)(h ? ph[v].y : ph[v].x);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) e[m] = (double)(h ? s[m][q].y : s[m][q].x);
+        const double een = elener_homog<V, NPH>(Pp, f, e, eg);
+        s_el += een;
+        if (h) eo.y = (T)een; else eo.x = (T)een;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const T g = (T)eg[v];
+          const T nf = (T)((double)(h ? fd[v].y : fd[v].x) + eg[v]);
+          if (h) { go[v].y = g; fd[v].y = nf; } else { go[v].x = g; fd[v].x = nf; }
+        }
+      }
+      ((C2<T>*)(el_en + row))[n] = eo;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        ((C2<T>*)(elgrad + (size_t)v * n3 + row))[n] = go[v];
+        ((C2<T>*)(fdiff + (size_t)v * n3 + row))[n] = fd[v];
+      }
+    }
+  }
+  const double rsum = block_reduce<0, NW>(s_el, sh);
+  if (w.t == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = rsum;
+}
+
 // --------------------------------------------------------------- elener
 // ModElastic.f:493-691 fused with f_diff += el_grad (Multi_MEUMAPPS.f:880).
 template <typename T, int V, int NPH, bool HOMOG>
@@ -563,6 +702,22 @@ k_elener(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __
   double s_el = 0.0;
   bool bad = false;
   GRID_STRIDE(pt, n3) {
+    if constexpr (HOMOG) {
+      double f[V], e[6], eg[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) f[v] = ld(phi, v * n3 + pt);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) e[m] = ld(strain, m * n3 + pt);
+      const double een = elener_homog<V, NPH>(P, f, e, eg);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        st(elgrad, v * n3 + pt, eg[v]);
+        st(fdiff, v * n3 + pt, ld(fdiff, v * n3 + pt) + eg[v]);
+      }
+      st(el_en, pt, een);
+      s_el += een;
+      continue;
+    }
     double f[V], Pv[V], ps = 0.0, te[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int v = 0; v < V; ++v) {
@@ -579,23 +734,7 @@ k_elener(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __
       eps[m] = ld(strain, m * n3 + pt) + w * P->e_mean[m] - w * te[m];
     }
     double el_mat, el_ppt[NPH + 1];
-    if constexpr (HOMOG) {
-      // the unguarded mixture sum P S + (1 - sum P) S is S: sig = C eps, and
-      // every phase's energy density equals the matrix one
-#pragma unroll
-      for (int a = 0; a < 6; ++a) {
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) s += P->cela[c * 6 + a] * eps[c];
-        sig[a] = s;
-      }
-      double e2 = 0.0;
-#pragma unroll
-      for (int a = 0; a < 6; ++a) e2 += sig[a] * eps[a];
-      el_mat = 0.5 * e2;
-#pragma unroll
-      for (int q = 2; q <= NPH; ++q) el_ppt[q] = el_mat;
-    } else {
+    {
       // the matrix term is unguarded here (ModElastic.f:597): with sum P > 1
       // the mixture can be indefinite, so this solve pivots
       double M[6][6];
@@ -880,8 +1019,11 @@ template <typename T> struct Kern {
   void (*zeroth)(dim3, hipStream_t, const DevParams*, const C2<T>*, C2<T>*, C2<T>*, size_t) = nullptr;
   // fused x pass + polarization stress (homogeneous moduli, supported x lengths only):
   // returns false when it has no kernel for this x length
-  bool (*xpolar)(bool launch, hipStream_t, const DevParams*, const T*, C2<T>*, size_t, size_t, const void*,
-                 const void*, int, int, int, T, double, const double*, const int*) = nullptr;
+  bool (*xpolar)(bool launch, hipStream_t, const DevParams*, const T*, const C2<T>*, C2<T>*, size_t, size_t,
+                 const void*, const void*, int, int, int, T, double, const double*, const int*) = nullptr;
+  // fused x c2r of the strain + elener (homogeneous moduli): nblk gets the workgroup count
+  bool (*xelener)(bool launch, hipStream_t, const DevParams*, const T*, const C2<T>*, size_t, size_t, const void*,
+                  const void*, int, int, int, T*, T*, T*, double*, const int*, int*) = nullptr;
 };
 
 // plan of the fused x kernel: six components of E points stay in registers
@@ -890,8 +1032,8 @@ using XPlan = std::conditional_t<sizeof(T) == 8, PlanLean4<L>, PlanLean<L>>;
 template <typename T, int L> constexpr bool xplan_ok() { return XPlan<T, L>::E <= (sizeof(T) == 8 ? 4 : 8); }
 
 template <typename T, int L, int V>
-static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* phi, C2<T>* work, size_t nc,
-                      size_t n3, const void* twL, const void* twN, int ny, int nz, T scale, double cfac,
+static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* in, C2<T>* work,
+                      size_t nc, size_t n3, const void* twL, const void* twN, int ny, int nz, T scale, double cfac,
                       const double* cfd, const int* gate) {
   if constexpr (!xplan_ok<T, L>()) {
     return false;
@@ -900,9 +1042,27 @@ static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* p
     using TL = TileLM<T, XP>;
     if (launch) {
       dim3 grid((ny + TL::B - 1) / TL::B, nz);
-      hipLaunchKernelGGL((k_x_polar<T, XP, V>), grid, dim3(TL::THREADS), 0, s, P, phi, work, nc, n3,
+      hipLaunchKernelGGL((k_x_polar<T, XP, V>), grid, dim3(TL::THREADS), 0, s, P, phi, in, work, nc, n3,
                          (const C2<T>*)twL, (const C2<T>*)twN, ny, scale, cfac, cfd, gate);
     }
+    return true;
+  }
+}
+
+template <typename T, int L, int V, int NPH>
+static bool go_xelener(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* sstrain, size_t nc,
+                       size_t n3, const void* twL, const void* twN, int ny, int nz, T* el_en, T* elgrad, T* fdiff,
+                       double* part, const int* gate, int* nblk) {
+  if constexpr (!xplan_ok<T, L>()) {
+    return false;
+  } else {
+    using XP = XPlan<T, L>;
+    using TL = TileLM<T, XP>;
+    dim3 grid((ny + TL::B - 1) / TL::B, nz);
+    if (nblk) *nblk = (int)(grid.x * grid.y);
+    if (launch)
+      hipLaunchKernelGGL((k_x_elener<T, XP, V, NPH>), grid, dim3(TL::THREADS), 0, s, P, phi, sstrain, nc, n3,
+                         (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, part, gate);
     return true;
   }
 }
@@ -932,12 +1092,25 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
   static void zeroth(dim3 g, hipStream_t s, const DevParams* P, const C2<T>* a, C2<T>* b, C2<T>* c, size_t n) {
     hipLaunchKernelGGL((k_ks_zeroth<T, V>), g, dim3(kBlock), 0, s, P, a, b, c, n);
   }
-  static bool xpolar(bool launch, hipStream_t s, const DevParams* P, const T* phi, C2<T>* work, size_t nc,
-                     size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, T scale, double cfac,
-                     const double* cfd, const int* gate) {
+  static bool xpolar(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* in, C2<T>* work,
+                     size_t nc, size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, T scale,
+                     double cfac, const double* cfd, const int* gate) {
     switch (nx / 2) {
 #define X(L_) \
-  case L_: return go_xpolar<T, L_, V>(launch, s, P, phi, work, nc, n3, twL, twN, ny, nz, scale, cfac, cfd, gate);
+  case L_: return go_xpolar<T, L_, V>(launch, s, P, phi, in, work, nc, n3, twL, twN, ny, nz, scale, cfac, cfd, gate);
+      X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+#undef X
+      default: return false;
+    }
+  }
+  static bool xelener(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* sstrain, size_t nc,
+                      size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, T* el_en, T* elgrad,
+                      T* fdiff, double* part, const int* gate, int* nblk) {
+    switch (nx / 2) {
+#define X(L_)                                                                                                  \
+  case L_:                                                                                                     \
+    return go_xelener<T, L_, V, NPH>(launch, s, P, phi, sstrain, nc, n3, twL, twN, ny, nz, el_en, elgrad, fdiff, \
+                                     part, gate, nblk);
       X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 #undef X
       default: return false;
@@ -947,6 +1120,7 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
     Kern<T> k;
     k.chem = chem; k.ppoly = ppoly; k.clamp = clamp; k.zeroth = zeroth;
     k.xpolar = homog ? xpolar : nullptr;
+    k.xelener = homog ? xelener : nullptr;
     k.polar = homog ? polar<true> : polar<false>;
     k.elener = homog ? elener<true> : elener<false>;
     return k;
@@ -991,6 +1165,10 @@ struct mm_ctx {
   // fused path: spectra between the x and y passes (S layout), kStageN of them;
   // the y pass moves them to and from the column-tiled layout of the z kernels
   void* sstage = nullptr;
+  // with the fused x path: the strain between steps, as x half-spectra (S
+  // layout after the inverse y pass, 6 components); c->strain is then filled
+  // from it only on demand (field access, checkpoints)
+  void* sstrain = nullptr;
   int nwork = 0, nxpol = 0;
   double* part = nullptr;
   double* dres = nullptr;
@@ -1169,6 +1347,8 @@ template <typename T> int uhat_s_to_z(mm_ctx* c, C2<T>* scratch3) {
   return 0;
 }
 
+template <typename T> int strain_from_real(mm_ctx* c);
+
 // ------------------------------------------------------------ elasticity
 // zeroth-order displacement at step == elastep0 (ModElastic.f:105-236)
 template <typename T> int elastic_zeroth(mm_ctx* c) {
@@ -1186,6 +1366,7 @@ template <typename T> int elastic_zeroth(mm_ctx* c) {
   MM_HIP(hipGetLastError());
   MM_TRY(inv<T>(c, wsp, wre, 6));
   MM_TRY(uhat_s_to_z<T>(c, work));
+  if (c->xpolar) MM_TRY(strain_from_real<T>(c));
   return 0;
 }
 
@@ -1207,8 +1388,8 @@ template <typename T> int elastic_iter(mm_ctx* c, double cfac, const double* cfa
     const T scale = (T)(1.0 / (double)(float)n3);
     c->probe.begin(MM_PROBE_X_POLAR, c->s);
     C2<T>* stg = (C2<T>*)c->sstage;
-    K.xpolar(true, c->s, c->dp, (const T*)c->phi, stg, nc, n3, c->fft.twLx, c->fft.twNx, c->fft.nx, c->fft.ny,
-             c->fft.nz, scale, cfac, cfac_dev, c->gate);
+    K.xpolar(true, c->s, c->dp, (const T*)c->phi, (const C2<T>*)c->sstrain, stg, nc, n3, c->fft.twLx, c->fft.twNx,
+             c->fft.nx, c->fft.ny, c->fft.nz, scale, cfac, cfac_dev, c->gate);
     c->probe.end(MM_PROBE_X_POLAR, c->s, (double)sizeof(T) * (24.0 * nc + (double)c->V * n3));
     MM_HIP(hipGetLastError());
     void* sp[6];
@@ -1253,11 +1434,11 @@ template <typename T> int elastic_strain_inverse(mm_ctx* c, bool* spectral) {
   T* wre[6];
   for (int m = 0; m < 6; ++m) { wsp[m] = work + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
   if (c->xpolar) {
-    // the strain stays as S spectra in the stage for the fused x kernel
+    // the strain stays as x half-spectra for the fused x kernels
     *spectral = true;
-    C2<T>* stg = (C2<T>*)c->sstage;
+    C2<T>* ss = (C2<T>*)c->sstrain;
     void* sp[6];
-    for (int m = 0; m < 6; ++m) sp[m] = stg + (size_t)m * nc;
+    for (int m = 0; m < 6; ++m) sp[m] = ss + (size_t)m * nc;
     return c->fft.y_pass_t<T>((void* const*)wsp, sp, 6, +1, 2, c->s);
   }
   *spectral = false;
@@ -1265,14 +1446,31 @@ template <typename T> int elastic_strain_inverse(mm_ctx* c, bool* spectral) {
   return inv<T>(c, wsp, wre, 6);
 }
 
-// the real strain fields from the spectra left by elastic_strain_inverse
-template <typename T> int elastic_strain_real(mm_ctx* c) {
+// the real strain from its x half-spectra, and back (scale 1/Nx: the x c2r
+// is unnormalised)
+template <typename T> int strain_to_real(mm_ctx* c) {
   const size_t n3 = c->n3, nc = c->nc;
-  C2<T>* stg = (C2<T>*)c->sstage;
-  void* wsp[6];
-  T* wre[6];
-  for (int m = 0; m < 6; ++m) { wsp[m] = stg + (size_t)m * nc; wre[m] = (T*)c->strain + (size_t)m * n3; }
-  return c->fft.x_c2r<T>(wsp, wre, 6, c->s);
+  C2<T>* ss = (C2<T>*)c->sstrain;
+  void* sp[6];
+  T* re[6];
+  for (int m = 0; m < 6; ++m) { sp[m] = ss + (size_t)m * nc; re[m] = (T*)c->strain + (size_t)m * n3; }
+  return c->fft.x_c2r<T>(sp, re, 6, c->s);
+}
+template <typename T> int strain_from_real(mm_ctx* c) {
+  const size_t n3 = c->n3, nc = c->nc;
+  C2<T>* ss = (C2<T>*)c->sstrain;
+  void* sp[6];
+  const T* re[6];
+  for (int m = 0; m < 6; ++m) { sp[m] = ss + (size_t)m * nc; re[m] = (const T*)c->strain + (size_t)m * n3; }
+  return c->fft.x_r2c<T>(re, sp, 6, c->s, (T)(1.0 / (double)c->fft.nx));
+}
+int strain_to_real_any(mm_ctx* c) {
+  if (!c->xpolar) return 0;
+  return c->prec == MM_FP64 ? strain_to_real<double>(c) : strain_to_real<float>(c);
+}
+int strain_from_real_any(mm_ctx* c) {
+  if (!c->xpolar) return 0;
+  return c->prec == MM_FP64 ? strain_from_real<double>(c) : strain_from_real<float>(c);
 }
 
 // eager iterations with host control, starting from the control state h
@@ -1285,7 +1483,6 @@ template <typename T> int elastic_loop_eager(mm_ctx* c, Ctl& h, bool spectral) {
     MM_HIP(hipEventSynchronize(c->ev));
     ctl_after_iter(h, c->hres + R_ERR0);
   }
-  if (spectral) MM_TRY(elastic_strain_real<T>(c));
   c->last_niter = h.total;
   c->last_cfac = h.cfac;
   return h.err;
@@ -1360,11 +1557,18 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   const dim3 gp(c->nblk_pw);
   if (elastic) {
     c->probe.begin(MM_PROBE_ELENER, c->s);
-    K.elener(gp, c->s, c->dp, phi, (T*)c->strain, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->derr, n3,
-             c->gate);
+    int nblk = c->nblk_pw;
+    if (c->xpolar) {
+      // the strain's x c2r fused with elener (the real strain is not formed)
+      K.xelener(true, c->s, c->dp, phi, (const C2<T>*)c->sstrain, nc, n3, c->fft.twLx, c->fft.twNx, c->fft.nx,
+                c->fft.ny, c->fft.nz, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->gate, &nblk);
+    } else {
+      K.elener(gp, c->s, c->dp, phi, (T*)c->strain, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->derr,
+               n3, c->gate);
+    }
     c->probe.end(MM_PROBE_ELENER, c->s, (double)n3 * sizeof(T) * (4 * c->V + 7));
     MM_HIP(hipGetLastError());
-    MM_TRY(finalize(c, c->nblk_pw, 1, {0}, R_EL));
+    MM_TRY(finalize(c, nblk, 1, {0}, R_EL));
     if (rec_fd) MM_HIP(hipMemcpyAsync(rec_fd, c->fdiff, (size_t)V * n3 * sizeof(T), hipMemcpyDeviceToDevice, c->s));
   } else {
     hipLaunchKernelGGL(k_fill<T>, gp, dim3(kBlock), 0, c->s, (T*)c->el_en, n3, (T)0, c->gate);
@@ -1492,7 +1696,7 @@ template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
     if (c->step == c->prm.elastep0) MM_TRY(elastic_zeroth<T>(c));
     Ctl h;
     ctl_start(h);
-    MM_TRY(elastic_loop_eager<T>(c, h, false));
+    MM_TRY(elastic_loop_eager<T>(c, h, c->xpolar));
   }
   MM_TRY(step_tail<T>(c, elastic));
   return step_collect(c, d);
@@ -1505,7 +1709,7 @@ template <typename T> int record_step(mm_ctx* c, double* stage) {
   MM_TRY(step_head<T>(c));
   hipLaunchKernelGGL(k_ctl_start, dim3(1), dim3(1), 0, c->s, c->ctl);
   MM_HIP(hipGetLastError());
-  bool spectral = false;
+  bool spectral = c->xpolar;
   for (int i = 0; i < c->graph_iters; ++i) {
     set_gate(c, &c->ctl->active);
     MM_TRY(elastic_iter<T>(c, 0.0, &c->ctl->cfac, spectral));
@@ -1514,10 +1718,6 @@ template <typename T> int record_step(mm_ctx* c, double* stage) {
     MM_HIP(hipGetLastError());
     set_gate(c, &c->ctl->ran);
     MM_TRY(elastic_strain_inverse<T>(c, &spectral));
-  }
-  if (spectral) {
-    set_gate(c, &c->ctl->done);
-    MM_TRY(elastic_strain_real<T>(c));
   }
   set_gate(c, &c->ctl->done);
   MM_TRY(step_tail<T>(c, true));
@@ -1536,7 +1736,7 @@ template <typename T> int finish_replay(mm_ctx* c, const double* st, mm_diag* d)
   std::memcpy(&h, st, sizeof(Ctl));
   if (!h.done) {
     set_gate(c, nullptr);
-    MM_TRY(elastic_loop_eager<T>(c, h, c->xpolar && c->graph_iters > 0));
+    MM_TRY(elastic_loop_eager<T>(c, h, c->xpolar));
     MM_TRY(step_tail<T>(c, true));
     return step_collect(c, d);
   }
@@ -1583,7 +1783,7 @@ void free_all(mm_ctx* c) {
   c->g = nullptr;
   void* bufs[] = {c->phi, c->cons, c->gphi, c->kdel, c->fdiff, c->flux, c->strain, c->xpol, c->el_en,
                   c->elgrad, c->sphi, c->suhat, c->swork, c->part, c->dres, c->derr, c->dp, c->ctl,
-                  c->dstage, c->sstage};
+                  c->dstage, c->sstage, c->sstrain};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
@@ -1658,6 +1858,10 @@ int upload(mm_ctx* c, const double* phi, const double* cons, const double* el_en
       h[5 * n3 + p] = g(1, 0, p) + g(0, 1, p);
     }
     MM_TRY(put_field<T>(c, c->strain, h));
+    if (c->xpolar) {
+      MM_TRY(strain_from_real<T>(c));
+      MM_HIP(hipStreamSynchronize(c->s));
+    }
   }
   if (u) {
     std::vector<double> h(u, u + 3 * n3);
@@ -1763,12 +1967,22 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
     const char* xf = std::getenv("MM_XFUSED");
     const bool xf_on = !(xf && xf[0] == '0');
     if (!rc && xf_on && c->zfused) {
+      // both fused x kernels exist for this x length, and the elener grid
+      // fits the partial-sum buffer
+      int nb = 0;
       if (precision == MM_FP64)
-        c->xpolar = c->k64.xpolar && c->k64.xpolar(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr,
-                                                    p->nx, p->ny, p->nz, 0.0, 0.0, nullptr, nullptr);
+        c->xpolar = c->k64.xpolar && c->k64.xelener &&
+                    c->k64.xpolar(false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx,
+                                  p->ny, p->nz, 0.0, 0.0, nullptr, nullptr) &&
+                    c->k64.xelener(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx, p->ny,
+                                   p->nz, nullptr, nullptr, nullptr, nullptr, nullptr, &nb);
       else
-        c->xpolar = c->k32.xpolar && c->k32.xpolar(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr,
-                                                    p->nx, p->ny, p->nz, 0.0f, 0.0, nullptr, nullptr);
+        c->xpolar = c->k32.xpolar && c->k32.xelener &&
+                    c->k32.xpolar(false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx,
+                                  p->ny, p->nz, 0.0f, 0.0, nullptr, nullptr) &&
+                    c->k32.xelener(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx, p->ny,
+                                   p->nz, nullptr, nullptr, nullptr, nullptr, nullptr, &nb);
+      if (nb > kMaxPartBlocks) c->xpolar = false;
     }
     const char* zv = std::getenv("MM_ZF_ELASTIC_VARIANT");
     if (zv) c->zf_el_var = std::max(0, std::min(kZfElasticVariants - 1, std::atoi(zv)));
@@ -1798,6 +2012,7 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   rc = rc ? rc : dalloc(&c->suhat, 3 * nc * cb);
   rc = rc ? rc : dalloc(&c->swork, c->nwork * nc * cb);
   if (c->zfused) rc = rc ? rc : dalloc(&c->sstage, (size_t)kStageN * nc * cb);
+  if (c->xpolar) rc = rc ? rc : dalloc(&c->sstrain, (size_t)6 * nc * cb);
   rc = rc ? rc : dalloc((void**)&c->part, (size_t)kMaxPartSlots * kMaxPartBlocks * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->dres, 64 * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->derr, 16);
@@ -1813,10 +2028,10 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   if (!rc && c->autotune) rc = c->fft.autotune(c->s);
   if (!rc) {
     // zero state so that a step without upload is well defined
-    void* z[] = {c->phi, c->cons, c->strain, c->el_en, c->elgrad, c->suhat};
-    size_t zb[] = {V * n3 * rb, E * n3 * rb, 6 * n3 * rb, n3 * rb, V * n3 * rb, 3 * nc * cb};
-    for (int i = 0; i < 6 && !rc; ++i)
-      if (hipMemset(z[i], 0, zb[i]) != hipSuccess) rc = MM_ERR_HIP;
+    void* z[] = {c->phi, c->cons, c->strain, c->el_en, c->elgrad, c->suhat, c->sstrain};
+    size_t zb[] = {V * n3 * rb, E * n3 * rb, 6 * n3 * rb, n3 * rb, V * n3 * rb, 3 * nc * cb, 6 * nc * cb};
+    for (int i = 0; i < 7 && !rc; ++i)
+      if (z[i] && hipMemset(z[i], 0, zb[i]) != hipSuccess) rc = MM_ERR_HIP;
   }
   if (rc) { free_all(c); delete c; return rc; }
   *out = c;
@@ -1868,7 +2083,14 @@ extern "C" int mm_field(mm_ctx* c, int32_t which, void** dev, int32_t* esz, int3
     case 1: p = c->cons; n = c->E; break;
     case 2: p = c->el_en; n = 1; break;
     case 3: p = c->fdiff; n = c->V; break;
-    case 4: p = c->strain; n = 6; break;
+    case 4:
+      // with the fused x path the strain lives as x half-spectra: form the
+      // real fields now (valid until the next step)
+      MM_TRY(strain_to_real_any(c));
+      MM_HIP(hipStreamSynchronize(c->s));
+      p = c->strain;
+      n = 6;
+      break;
     case 5: p = c->gphi; n = 3 * c->V; break;
     case 6: p = c->kdel; n = c->V; break;
     case 7: p = c->elgrad; n = c->V; break;
