@@ -65,12 +65,18 @@ template <int L> struct ZPlan<L, 2> { using type = Plan<L>; };
 // memory.  The twiddle table is staged in LDS once per workgroup: the
 // per-stage twiddle reads of the 12 transforms of the elastic operation
 // would otherwise be a chain of dependent global loads.
+// The model parameters come in as a separate restrict pointer (Pr, the same
+// block as op.P): nothing the kernel stores can alias them, so their reads
+// stay scalar loads into SGPRs instead of vector reloads after every store.
+// Loads of a column past ncols read column 0 (in bounds, never stored), so
+// that no load sits behind a branch.
 template <typename T, int L, class Op>
 __global__ void __launch_bounds__((TileZF<T, typename ZPlan<L, Op::PLAN>::type, Op::MAXT>::THREADS),
                                   (TileZF<T, typename ZPlan<L, Op::PLAN>::type, Op::MAXT>::template minw<Op::MINW>()))
-k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
+k_z_fused(Op op, const DevParams* __restrict__ Pr, const C2<T>* __restrict__ twL, int ncols) {
   using P = typename ZPlan<L, Op::PLAN>::type;
   using TL = TileZF<T, P, Op::MAXT>;
+  op.P = Pr;
   if (gated_off(op.gate)) return;
   constexpr int TPL = P::TPL, E = P::E, B = TL::B;
   static_assert(B % kTW == 0, "a workgroup owns whole tiles");
@@ -89,7 +95,7 @@ k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
   {
     const C2<T>* in = op.input(g, 0);
 #pragma unroll
-    for (int q = 0; q < E; ++q) vn[q] = valid ? in[zcol(col, tl + TPL * q, L)] : mk<T>(0, 0);
+    for (int q = 0; q < E; ++q) vn[q] = in[zcol(colc, tl + TPL * q, L)];
   }
 #pragma unroll 1
   for (int f = 0; f < Op::NIN; ++f) {
@@ -99,7 +105,7 @@ k_z_fused(Op op, const C2<T>* __restrict__ twL, int ncols) {
     if (f + 1 < Op::NIN) {
       const C2<T>* in = op.input(g, f + 1);
 #pragma unroll
-      for (int q = 0; q < E; ++q) vn[q] = valid ? in[zcol(col, tl + TPL * q, L)] : mk<T>(0, 0);
+      for (int q = 0; q < E; ++q) vn[q] = in[zcol(colc, tl + TPL * q, L)];
     }
     if (f > 0) __syncthreads();
     line_fft<T, -1, P>(v, tl, lds, idx, tw);
@@ -233,11 +239,9 @@ template <typename T> struct OpElastic {
       default: absorb_f<5, E>(st, v, ck, tl, TPL); break;
     }
   }
+  // every solve first, then one batch of u_hat loads and one of stores
   template <int E>
-  __device__ void finish(State<E>& st, int, const ColK& ck, int col, int tl, int TPL, int ncols, bool valid) const {
-#pragma unroll
-    for (int m = 0; m < 6; ++m) st.acc[m] = 0.0;
-    const double w = (ck.kx == 0 || 2 * ck.kx == P->nx) ? 1.0 : 2.0;
+  __device__ void finish(State<E>& st, int, const ColK& ck, int col, int tl, int TPL, int, bool valid) const {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int kz = tl + TPL * q;
@@ -253,18 +257,35 @@ template <typename T> struct OpElastic {
       }
 #pragma unroll
       for (int i = 0; i < 3; ++i) { st.rr[i][q] = u.re[i]; st.ri[i][q] = u.im[i]; }
-      if (valid) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          C2<T>* uh = uhat + (size_t)i * nc + zcol(col, kz, P->nz);
-          const C2<T> uo = *uh;
-          const double dr = u.re[i] - (double)uo.x, di = u.im[i] - (double)uo.y;
-          st.acc[2 * i] += w * (dr * dr + di * di);
-          st.acc[2 * i + 1] += w * ((double)uo.x * (double)uo.x + (double)uo.y * (double)uo.y);
-          *uh = mk<T>((T)u.re[i], (T)u.im[i]);
-        }
-      }
     }
+    const int nz = P->nz;
+    C2<T> uo[3][E];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int q = 0; q < E; ++q) uo[i][q] = uhat[(size_t)i * nc + zcol(col, tl + TPL * q, nz)];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) st.acc[m] = 0.0;
+    const double w = (ck.kx == 0 || 2 * ck.kx == P->nx) ? 1.0 : 2.0;
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double ur = (double)uo[i][q].x, ui = (double)uo[i][q].y;
+        const double dr = st.rr[i][q] - ur, di = st.ri[i][q] - ui;
+        st.acc[2 * i] += w * (dr * dr + di * di);
+        st.acc[2 * i + 1] += w * (ur * ur + ui * ui);
+      }
+    if (!valid) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) st.acc[m] = 0.0;
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int q = 0; q < E; ++q)
+        uhat[(size_t)i * nc + zcol(col, tl + TPL * q, nz)] = mk<T>((T)st.rr[i][q], (T)st.ri[i][q]);
   }
   // e11, e22, e33, g23, g13, g12 = i kf_A u_B (+ i kf_A2 u_B2)
   template <int O, int E>
@@ -389,7 +410,7 @@ template <typename T> struct OpAC {
     const C2<T>* src = sphi + (size_t)g * nc;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const C2<T> s = valid ? src[zcol(col, tl + TPL * q, P->nz)] : mk<T>(0, 0);
+      const C2<T> s = src[zcol(col, tl + TPL * q, P->nz)];
       st.pr[q] = (double)s.x;
       st.pi[q] = (double)s.y;
     }

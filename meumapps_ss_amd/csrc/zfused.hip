@@ -11,7 +11,7 @@ static int go_zf(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int* n
   using TL = TileZF<T, typename ZPlan<L, Op::PLAN>::type, Op::MAXT>;
   const int ncols = f.H * f.ny;
   dim3 grid((ncols + TL::B - 1) / TL::B, ngroups);
-  hipLaunchKernelGGL((k_z_fused<T, L, Op>), grid, dim3(TL::THREADS), 0, s, op, (const C2<T>*)f.twNz, ncols);
+  hipLaunchKernelGGL((k_z_fused<T, L, Op>), grid, dim3(TL::THREADS), 0, s, op, op.P, (const C2<T>*)f.twNz, ncols);
   if (nblk) *nblk = (int)grid.x;
   return 0;
 }
