@@ -444,26 +444,40 @@ template <typename T, class P> struct XRows {
 };
 
 // c2r of six x half-spectra (in + m*nc, layout S after the inverse y pass) into
-// s[m][q] = (x[2n], x[2n+1]); imaginary parts of X_0 and X_L are ignored
+// s[m][q] = (x[2n], x[2n+1]); imaginary parts of X_0 and X_L are ignored.
+// The loads of component m+1 are issued before component m is transformed;
+// they are unconditional (indices clamped into the array; rows past ny are
+// computed but never stored), so that no load waits behind a branch.
 template <typename T, class P>
 __device__ __forceinline__ void x_c2r_six(const XRows<T, P>& w, const C2<T>* in, size_t nc, int ny, C2<T>* lds,
                                           const C2<T>* twl, const C2<T>* twn, C2<T> (&s)[6][P::E]) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
   const IdxLineMajor idx{w.bl * LP};
-#pragma unroll
-  for (int m = 0; m < 6; ++m) {
+  const size_t base = (size_t)w.z * (L + 1) * ny + (size_t)(w.yg < ny ? w.yg : ny - 1);
+  // point i < E is kx = r + TPL i (< L); point E is kx = L, kept by r == 0 only
+  // (component-wise copies: a whole-struct copy of double2 into the register
+  // sets becomes a memcpy that keeps them in scratch memory)
+  auto load = [&](int m, C2<T>(&x)[E + 1]) {
     const C2<T>* src = in + (size_t)m * nc;
-    C2<T> x[E + 1];
 #pragma unroll
     for (int i = 0; i <= E; ++i) {
-      const int kx = w.r + TPL * i;
-      x[i] = (w.yg < ny && kx <= L) ? src[((size_t)w.z * (L + 1) + kx) * ny + w.yg] : mk<T>(0, 0);
+      const C2<T>* p = src + base + (size_t)(i < E ? w.r + TPL * i : L) * ny;
+      x[i] = mk<T>(p->x, p->y);
     }
+  };
+  auto c2r = [&](int m, const C2<T>(&x)[E + 1]) {
     if (m > 0) __syncthreads();
 #pragma unroll
-    for (int i = 0; i <= E; ++i) {
+    for (int i = 0; i < E; ++i) {
       const int kx = w.r + TPL * i;
-      if (kx <= L) lds[w.bg * LP + kx + (kx >> 4)] = x[i];
+      C2<T>* d = lds + w.bg * LP + kx + (kx >> 4);
+      d->x = x[i].x;
+      d->y = x[i].y;
+    }
+    if (w.r == 0) {
+      C2<T>* d = lds + w.bg * LP + L + (L >> 4);
+      d->x = x[E].x;
+      d->y = x[E].y;
     }
     __syncthreads();
     C2<T> v[E];
@@ -483,7 +497,21 @@ __device__ __forceinline__ void x_c2r_six(const XRows<T, P>& w, const C2<T>* in,
     line_fft<T, +1, P>(v, w.tl, lds, idx, twl);
 #pragma unroll
     for (int q = 0; q < E; ++q) s[m][q] = v[q];
-  }
+  };
+  // two register sets in turn: component m+1 loads while m is transformed
+  C2<T> xa[E + 1], xb[E + 1];
+  load(0, xa);
+  load(1, xb);
+  c2r(0, xa);
+  load(2, xa);
+  c2r(1, xb);
+  load(3, xb);
+  c2r(2, xa);
+  load(4, xa);
+  c2r(3, xb);
+  load(5, xb);
+  c2r(4, xa);
+  c2r(5, xb);
 }
 
 // ------------------------------------ fused x pass with the polarization stress
@@ -511,29 +539,33 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
   for (int e = t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
   __syncthreads();
   const IdxLineMajor idx{w.bl * LP};
+  // phi of this thread's points, loaded ahead of the transforms
+  C2<T> ph[E][V];
+  {
+    const size_t row = ((size_t)z * ny + (yl < ny ? yl : ny - 1)) * (size_t)(2 * L);
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+#pragma unroll
+      for (int v = 0; v < V; ++v) ph[q][v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[tl + TPL * q];
+  }
   C2<T> s[6][E];
   // 1. c2r of the six strain components (in; may be the same buffer as work)
   x_c2r_six<T, P>(w, in, nc, ny, lds, twl, twn, s);
   // 2. polarization stress at the 2E points of this thread
   if (yl < ny) {
     const double cfac = cfac_dev ? *cfac_dev : cfac_arg;
-    const size_t row = ((size_t)z * ny + yl) * (size_t)(2 * L);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const int n = tl + TPL * q;
-      C2<T> ph[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) ph[v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[n];
       double f[V], e[6], xo[6];
 #pragma unroll
-      for (int v = 0; v < V; ++v) f[v] = (double)ph[v].x;
+      for (int v = 0; v < V; ++v) f[v] = (double)ph[q][v].x;
 #pragma unroll
       for (int m = 0; m < 6; ++m) e[m] = (double)s[m][q].x;
       polar_homog<V>(Pp, f, e, cfac, xo);
 #pragma unroll
       for (int m = 0; m < 6; ++m) s[m][q].x = (T)xo[m];
 #pragma unroll
-      for (int v = 0; v < V; ++v) f[v] = (double)ph[v].y;
+      for (int v = 0; v < V; ++v) f[v] = (double)ph[q][v].y;
 #pragma unroll
       for (int m = 0; m < 6; ++m) e[m] = (double)s[m][q].y;
       polar_homog<V>(Pp, f, e, cfac, xo);
@@ -649,13 +681,14 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
   __syncthreads();
   C2<T> s[6][E];
   x_c2r_six<T, P>(w, sstrain, nc, ny, lds, twl, twn, s);
+  // rows past ny read row ny-1 and store nothing
+  const size_t row = ((size_t)w.z * ny + (w.yl < ny ? w.yl : ny - 1)) * (size_t)(2 * L);
   double s_el = 0.0;
   if (w.yl < ny) {
-    const size_t row = ((size_t)w.z * ny + w.yl) * (size_t)(2 * L);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int n = w.tl + TPL * q;
-      C2<T> ph[V], fd[V], eo, go[V];
+      C2<T> eo, go[V], ph[V], fd[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         ph[v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[n];
