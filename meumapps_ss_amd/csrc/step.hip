@@ -1420,7 +1420,9 @@ template <typename T> int elastic_iter(mm_ctx* c, double cfac, const double* cfa
   if (spectral) {
     const T scale = (T)(1.0 / (double)(float)n3);
     c->probe.begin(MM_PROBE_X_POLAR, c->s);
-    C2<T>* stg = (C2<T>*)c->sstage;
+    // in place: the strain half-spectra are replaced by the polarization ones
+    // (the inverse pass of this iteration writes the new strain back)
+    C2<T>* stg = (C2<T>*)c->sstrain;
     K.xpolar(true, c->s, c->dp, (const T*)c->phi, (const C2<T>*)c->sstrain, stg, nc, n3, c->fft.twLx, c->fft.twNx,
              c->fft.nx, c->fft.ny, c->fft.nz, scale, cfac, cfac_dev, c->gate);
     c->probe.end(MM_PROBE_X_POLAR, c->s, (double)sizeof(T) * (24.0 * nc + (double)c->V * n3));
