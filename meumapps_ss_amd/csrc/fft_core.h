@@ -196,9 +196,26 @@ template <class P, int S> struct StageR {
                                      : P::R0 * P::R1 * P::R2 * P::R3;
 };
 
+// Lines held by whole wavefronts: when a line's TPL threads are consecutive
+// (thread t works on line t / TPL) and TPL divides 64, no line straddles two
+// wavefronts, and the LDS exchanges between the stages of a line need only
+// wavefront-level ordering (a wavefront's LDS instructions execute in order),
+// not a workgroup barrier.
+template <class P> constexpr bool wave_lines() { return P::TPL <= 64 && 64 % P::TPL == 0; }
+template <bool WAVE> __device__ __forceinline__ void line_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham stage. `idx(e)` maps a line position to an LDS slot of this
 // thread's line; tw is the forward table exp(-2 pi i e / L), e in [0, L).
-template <typename T, int DIR, class P, int S, class Idx>
+// WAVE: the lines are held by whole wavefronts (wave_lines<P>()).
+template <typename T, int DIR, class P, int S, class Idx, bool WAVE = false>
 __device__ __forceinline__ void stage(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx,
                                       const C2<T>* __restrict__ tw) {
   constexpr int R = StageR<P, S>::R;
@@ -210,7 +227,7 @@ __device__ __forceinline__ void stage(C2<T> (&v)[P::E], int tl, C2<T>* lds, cons
     // gather this stage's inputs j + m*L/R = tl + TPL*(i + m*NB) from LDS
 #pragma unroll
     for (int q = 0; q < P::E; ++q) v[q] = lds[idx(tl + TPL * q)];
-    if constexpr (!last) __syncthreads();
+    if constexpr (!last) line_sync<WAVE>();
   }
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
@@ -237,26 +254,27 @@ __device__ __forceinline__ void stage(C2<T> (&v)[P::E], int tl, C2<T>* lds, cons
       for (int m = 0; m < R; ++m) lds[idx(ob + m * Ls)] = a[m];
     }
   }
-  if constexpr (!last) __syncthreads();
+  if constexpr (!last) line_sync<WAVE>();
 }
 
-template <typename T, int DIR, class P, int S, class Idx>
+template <typename T, int DIR, class P, int S, class Idx, bool WAVE = false>
 __device__ __forceinline__ void stages(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx,
                                        const C2<T>* __restrict__ tw) {
   if constexpr (S < P::NS) {
-    stage<T, DIR, P, S, Idx>(v, tl, lds, idx, tw);
-    stages<T, DIR, P, S + 1, Idx>(v, tl, lds, idx, tw);
+    stage<T, DIR, P, S, Idx, WAVE>(v, tl, lds, idx, tw);
+    stages<T, DIR, P, S + 1, Idx, WAVE>(v, tl, lds, idx, tw);
   }
 }
 
 // Transform one line held as v[q] = x[tl + TPL*q]; on return v[q] = X[tl + TPL*q].
 // DIR = -1: X_k = sum x_n exp(-2 pi i nk/L); DIR = +1 the unnormalised inverse.
 // When the plan has more than one stage the LDS slots of this line are
-// overwritten and the caller must barrier before reusing them.
-template <typename T, int DIR, class P, class Idx>
+// overwritten and the caller must barrier before reusing them.  WAVE = true
+// only for lines held by whole wavefronts (wave_lines<P>()).
+template <typename T, int DIR, class P, class Idx, bool WAVE = false>
 __device__ __forceinline__ void line_fft(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx,
                                          const C2<T>* __restrict__ tw) {
-  stages<T, DIR, P, 0, Idx>(v, tl, lds, idx, tw);
+  stages<T, DIR, P, 0, Idx, WAVE>(v, tl, lds, idx, tw);
 }
 
 // LDS slot maps

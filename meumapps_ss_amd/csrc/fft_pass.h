@@ -78,8 +78,8 @@ k_x_r2c(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ tw
       for (int q = 0; q < E; ++q) v[q] = mk<T>(0, 0);
     }
     IdxLineMajor idx{b * LP};
-    line_fft<T, -1, P>(v, tl, lds, idx, twL);
-    if constexpr (P::NS > 1) __syncthreads();
+    line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, twL);
+    if constexpr (P::NS > 1) line_sync<wave_lines<P>()>();
 #pragma unroll
     for (int q = 0; q < E; ++q) lds[idx(tl + TPL * q)] = v[q];
   }
@@ -147,8 +147,8 @@ k_x_c2r(Batch<T> bt, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ tw
     C2<T> od = cmul<T>(dd, w);
     v[q] = mk<T>(ev.x - od.y, ev.y + od.x);  // ev + i*od
   }
-  __syncthreads();
-  line_fft<T, +1, P>(v, tl, lds, idx, twL);
+  line_sync<wave_lines<P>()>();  // the reads above touch this line only
+  line_fft<T, +1, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, twL);
   if (y < ny) {
     C2<T>* row = (C2<T>*)(out + ((size_t)z * ny + y) * (size_t)(2 * L));
 #pragma unroll
@@ -178,7 +178,7 @@ k_y_pass(Batch<T> bt, const C2<T>* __restrict__ twL, int nlines) {
     for (int q = 0; q < E; ++q) v[q] = mk<T>(0, 0);
   }
   IdxLineMajor idx{b * LP};
-  line_fft<T, DIR, P>(v, tl, lds, idx, twL);
+  line_fft<T, DIR, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, twL);
   if (line < nlines) {
 #pragma unroll
     for (int q = 0; q < E; ++q) out[tl + TPL * q] = v[q];
@@ -233,7 +233,7 @@ k_y_pass_t(Batch<T> bt, const C2<T>* __restrict__ twL, int nz, int H) {
     for (int q = 0; q < E; ++q) v[q] = mk<T>(0, 0);
   }
   IdxLineMajor idx{b * LP};
-  line_fft<T, DIR, P>(v, tl, lds, idx, tw);
+  line_fft<T, DIR, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, tw);
   if (ok) {
 #pragma unroll
     for (int q = 0; q < E; ++q) {

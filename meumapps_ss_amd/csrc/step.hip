@@ -493,8 +493,8 @@ __device__ __forceinline__ void x_c2r_six(const XRows<T, P>& w, const C2<T>* in,
       C2<T> od = cmul<T>(dd, tw);
       v[q] = mk<T>(ev.x - od.y, ev.y + od.x);
     }
-    __syncthreads();
-    line_fft<T, +1, P>(v, w.tl, lds, idx, twl);
+    line_sync<wave_lines<P>()>();  // the reads above touch this line only
+    line_fft<T, +1, P, IdxLineMajor, wave_lines<P>()>(v, w.tl, lds, idx, twl);
 #pragma unroll
     for (int q = 0; q < E; ++q) s[m][q] = v[q];
   };
@@ -581,8 +581,8 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = s[m][q];
     __syncthreads();
-    line_fft<T, -1, P>(v, tl, lds, idx, twl);
-    if constexpr (P::NS > 1) __syncthreads();
+    line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, twl);
+    if constexpr (P::NS > 1) line_sync<wave_lines<P>()>();
 #pragma unroll
     for (int q = 0; q < E; ++q) lds[idx(tl + TPL * q)] = v[q];
     __syncthreads();
