@@ -193,7 +193,8 @@ void mm_batch_destroy(mm_batch *batch);
 /* Device-resident field access for in-process callers (torch, tests):
  * returns the device address of one SoA field and its element size (8 or 4).
  * which: 0 phi, 1 cons, 2 el_en, 3 f_diff (last step), 4 strain (6 fields:
- * e11, e22, e33, 2e23, 2e13, 2e12 incl. e_mean), 5 grad_phi (3V fields),
+ * e11, e22, e33, 2e23, 2e13, 2e12 of the displacement u; e_mean is not
+ * included), 5 grad_phi (3V fields),
  * 6 k_del_sq (V fields), 7 el_grad (V fields).  With the fused x path
  * (homogeneous moduli) the strain is kept as x half-spectra between steps:
  * which = 4 forms the real fields on this call, valid until the next step. */
