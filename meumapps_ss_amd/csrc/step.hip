@@ -443,14 +443,23 @@ template <typename T, class P> struct XRows {
   }
 };
 
-// c2r of six x half-spectra (in + m*nc, layout S after the inverse y pass) into
+// two register sets in turn: component M+1 loads (into xb) while M (in xa)
+// is transformed
+template <int NC, int M, class LD, class CV, class X>
+__device__ __forceinline__ void x_c2r_pipe(LD& load, CV& c2r, X& xa, X& xb) {
+  if constexpr (M + 1 < NC) load(M + 1, xb);
+  c2r(M, xa);
+  if constexpr (M + 1 < NC) x_c2r_pipe<NC, M + 1>(load, c2r, xb, xa);
+}
+
+// c2r of NC x half-spectra (in + m*nc, layout S after the inverse y pass) into
 // s[m][q] = (x[2n], x[2n+1]); imaginary parts of X_0 and X_L are ignored.
 // The loads of component m+1 are issued before component m is transformed;
 // they are unconditional (indices clamped into the array; rows past ny are
 // computed but never stored), so that no load waits behind a branch.
-template <typename T, class P>
-__device__ __forceinline__ void x_c2r_six(const XRows<T, P>& w, const C2<T>* in, size_t nc, int ny, C2<T>* lds,
-                                          const C2<T>* twl, const C2<T>* twn, C2<T> (&s)[6][P::E]) {
+template <typename T, class P, int NC>
+__device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, size_t nc, int ny, C2<T>* lds,
+                                        const C2<T>* twl, const C2<T>* twn, C2<T> (&s)[NC][P::E]) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
   const IdxLineMajor idx{w.bl * LP};
   const size_t base = (size_t)w.z * (L + 1) * ny + (size_t)(w.yg < ny ? w.yg : ny - 1);
@@ -498,20 +507,9 @@ __device__ __forceinline__ void x_c2r_six(const XRows<T, P>& w, const C2<T>* in,
 #pragma unroll
     for (int q = 0; q < E; ++q) s[m][q] = v[q];
   };
-  // two register sets in turn: component m+1 loads while m is transformed
   C2<T> xa[E + 1], xb[E + 1];
   load(0, xa);
-  load(1, xb);
-  c2r(0, xa);
-  load(2, xa);
-  c2r(1, xb);
-  load(3, xb);
-  c2r(2, xa);
-  load(4, xa);
-  c2r(3, xb);
-  load(5, xb);
-  c2r(4, xa);
-  c2r(5, xb);
+  x_c2r_pipe<NC, 0>(load, c2r, xa, xb);
 }
 
 // ------------------------------------ fused x pass with the polarization stress
@@ -550,7 +548,7 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
   }
   C2<T> s[6][E];
   // 1. c2r of the six strain components (in; may be the same buffer as work)
-  x_c2r_six<T, P>(w, in, nc, ny, lds, twl, twn, s);
+  x_c2r_n<T, P, 6>(w, in, nc, ny, lds, twl, twn, s);
   // 2. polarization stress at the 2E points of this thread
   if (yl < ny) {
     const double cfac = cfac_dev ? *cfac_dev : cfac_arg;
@@ -680,7 +678,7 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
   for (int e = w.t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
   __syncthreads();
   C2<T> s[6][E];
-  x_c2r_six<T, P>(w, sstrain, nc, ny, lds, twl, twn, s);
+  x_c2r_n<T, P, 6>(w, sstrain, nc, ny, lds, twl, twn, s);
   // rows past ny read row ny-1 and store nothing
   const size_t row = ((size_t)w.z * ny + (w.yl < ny ? w.yl : ny - 1)) * (size_t)(2 * L);
   double s_el = 0.0;
@@ -721,6 +719,87 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
   }
   const double rsum = block_reduce<0, NW>(s_el, sh);
   if (w.t == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = rsum;
+}
+
+// ------------------------------- Allen-Cahn inverse x pass fused with the clamp
+// Multi_MEUMAPPS.f:1001-1043: the V phi half-spectra left by the inverse y
+// pass are transformed to real rows in registers, clamped to [0, 1] and
+// written once; the check-loop partial sums of k_clamp (slots 0..E-1 sum of
+// the stored compositions, E min c, E+1 max c, E+2 max phi, E+3 sum of phi > 0)
+// go to part[slot * nblk + b], b = blockIdx.y * gridDim.x + blockIdx.x.
+template <typename T, class P, int V, int E>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
+k_x_clamp(const DevParams* __restrict__ Pp, const C2<T>* __restrict__ in, size_t nc, T* __restrict__ phi,
+          const T* __restrict__ cons, size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny,
+          double* __restrict__ part, const int* __restrict__ gate) {
+  constexpr int L = P::L, TPL = P::TPL, E_ = P::E, NW = (TileLM<T, P>::THREADS + 63) / 64;
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  __shared__ C2<T> twl[L], twn[2 * L];
+  __shared__ double sh[NW];
+  if (gated_off(gate)) return;
+  const XRows<T, P> w(ny);
+  for (int e = w.t; e < L; e += TileLM<T, P>::THREADS) twl[e] = twL[e];
+  for (int e = w.t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
+  __syncthreads();
+  C2<T> s[V][E_];
+  x_c2r_n<T, P, V>(w, in, nc, ny, lds, twl, twn, s);
+  double cs[E], cmin = HUGE_VAL, cmax = -HUGE_VAL, pmax = -HUGE_VAL, sf = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) cs[e] = 0.0;
+  if (w.yl < ny) {
+    const size_t row = ((size_t)w.z * ny + w.yl) * (size_t)(2 * L);
+    C2<T> cv[E][E_];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+#pragma unroll
+      for (int q = 0; q < E_; ++q) {
+        const C2<T>* pc = (const C2<T>*)(cons + (size_t)e * n3 + row) + (w.tl + TPL * q);
+        cv[e][q] = mk<T>(pc->x, pc->y);
+      }
+#pragma unroll
+    for (int q = 0; q < E_; ++q) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double raw = (double)(h ? cv[e][q].y : cv[e][q].x);
+          const double c = Pp->cbar[e] + raw;
+          if (c < cmin) cmin = c;
+          if (c > cmax) cmax = c;
+          cs[e] += raw;
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          double f = (double)(h ? s[v][q].y : s[v][q].x);
+          if (f < 0.0) f = 0.0;
+          if (f > 1.0) f = 1.0;
+          const T ft = (T)f;
+          if (h) s[v][q].y = ft; else s[v][q].x = ft;
+          f = (double)ft;
+          if (f > pmax) pmax = f;
+          if (f > 0.0) sf = sf + f;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        C2<T>* pp = (C2<T>*)(phi + (size_t)v * n3 + row) + (w.tl + TPL * q);
+        pp->x = s[v][q].x;
+        pp->y = s[v][q].y;
+      }
+    }
+  }
+  const size_t nblk = (size_t)gridDim.x * gridDim.y, b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  double r[E + 4];
+#pragma unroll
+  for (int e = 0; e < E; ++e) r[e] = block_reduce<0, NW>(cs[e], sh);
+  r[E] = block_reduce<1, NW>(cmin, sh);
+  r[E + 1] = block_reduce<2, NW>(cmax, sh);
+  r[E + 2] = block_reduce<2, NW>(pmax, sh);
+  r[E + 3] = block_reduce<0, NW>(sf, sh);
+  if (w.t == 0) {
+#pragma unroll
+    for (int k = 0; k < E + 4; ++k) part[(size_t)k * nblk + b] = r[k];
+  }
 }
 
 // --------------------------------------------------------------- elener
@@ -1057,6 +1136,9 @@ template <typename T> struct Kern {
   // fused x c2r of the strain + elener (homogeneous moduli): nblk gets the workgroup count
   bool (*xelener)(bool launch, hipStream_t, const DevParams*, const T*, const C2<T>*, size_t, size_t, const void*,
                   const void*, int, int, int, T*, T*, T*, double*, const int*, int*) = nullptr;
+  // Allen-Cahn inverse x pass + clamp + check-loop sums: nblk gets the workgroup count
+  bool (*xclamp)(bool launch, hipStream_t, const DevParams*, const C2<T>*, size_t, T*, const T*, size_t, const void*,
+                 const void*, int, int, int, double*, const int*, int*) = nullptr;
 };
 
 // plan of the fused x kernel: six components of E points stay in registers
@@ -1096,6 +1178,24 @@ static bool go_xelener(bool launch, hipStream_t s, const DevParams* P, const T* 
     if (launch)
       hipLaunchKernelGGL((k_x_elener<T, XP, V, NPH>), grid, dim3(TL::THREADS), 0, s, P, phi, sstrain, nc, n3,
                          (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, part, gate);
+    return true;
+  }
+}
+
+template <typename T, int L, int V, int E>
+static bool go_xclamp(bool launch, hipStream_t s, const DevParams* P, const C2<T>* in, size_t nc, T* phi,
+                      const T* cons, size_t n3, const void* twL, const void* twN, int ny, int nz, double* part,
+                      const int* gate, int* nblk) {
+  if constexpr (!xplan_ok<T, L>()) {
+    return false;
+  } else {
+    using XP = XPlan<T, L>;
+    using TL = TileLM<T, XP>;
+    dim3 grid((ny + TL::B - 1) / TL::B, nz);
+    if (nblk) *nblk = (int)(grid.x * grid.y);
+    if (launch)
+      hipLaunchKernelGGL((k_x_clamp<T, XP, V, E>), grid, dim3(TL::THREADS), 0, s, P, in, nc, phi, cons, n3,
+                         (const C2<T>*)twL, (const C2<T>*)twN, ny, part, gate);
     return true;
   }
 }
@@ -1149,8 +1249,21 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
       default: return false;
     }
   }
+  static bool xclamp(bool launch, hipStream_t s, const DevParams* P, const C2<T>* in, size_t nc, T* phi,
+                     const T* cons, size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, double* part,
+                     const int* gate, int* nblk) {
+    switch (nx / 2) {
+#define X(L_)                                                                                                  \
+  case L_:                                                                                                     \
+    return go_xclamp<T, L_, V, E>(launch, s, P, in, nc, phi, cons, n3, twL, twN, ny, nz, part, gate, nblk);
+      X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+#undef X
+      default: return false;
+    }
+  }
   static Kern<T> table(bool homog) {
     Kern<T> k;
+    k.xclamp = xclamp;
     k.chem = chem; k.ppoly = ppoly; k.clamp = clamp; k.zeroth = zeroth;
     k.xpolar = homog ? xpolar : nullptr;
     k.xelener = homog ? xelener : nullptr;
@@ -1219,6 +1332,9 @@ struct mm_ctx {
   // stress (needs zfused, homogeneous moduli and a fused kernel for Nx;
   // MM_XFUSED=0 switches it off)
   bool xpolar = false;
+  // the Allen-Cahn inverse x pass runs fused with the clamp (needs zfused and a
+  // fused kernel for Nx; MM_XFUSED=0 switches it off)
+  bool xclamp = false;
   int last_niter = 0;
   double last_cfac = 1.0;
   // graph mode: the regular step (step > elastep0) recorded once and replayed
@@ -1579,6 +1695,19 @@ template <typename T> int step_head(mm_ctx* c) {
   return 0;
 }
 
+// check-loop sums of the clamp from nblk partials per slot
+int finalize_clamp(mm_ctx* c, int nblk) {
+  const int E = c->E;
+  FinalizeOps fo;
+  for (int e = 0; e < E; ++e) fo.op[e] = 0;
+  fo.op[E] = 1; fo.op[E + 1] = 2; fo.op[E + 2] = 2; fo.op[E + 3] = 0;
+  for (int i = E + 4; i < 8; ++i) fo.op[i] = 0;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, c->s, c->part, nblk, E + 4, fo, c->dres + R_CLAMP,
+                     c->gate);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
 // tail: elener (when the elastic solve ran), Cahn-Hilliard, Allen-Cahn, clamp
 // rec_fd / rec_phipre (FP64 tape): copies of f_diff after elener and of phi
 // before the clamp
@@ -1655,6 +1784,19 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
       opac.gate = c->gate;
       MM_TRY(launch_zfused<T>(c->fft, opac, V, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 3 * V, nullptr));
       for (int v = 0; v < V; ++v) { sp[v] = work + (size_t)v * nc; rp[v] = phi + (size_t)v * n3; }
+      if (c->xclamp && !rec_phipre) {
+        // inverse y pass into the stage; the x pass, the clamp and the
+        // check-loop sums in one kernel (phi written once)
+        C2<T>* stg = (C2<T>*)c->sstage;
+        void* st[kStageN];
+        for (int v = 0; v < V; ++v) st[v] = stg + (size_t)v * nc;
+        MM_TRY(c->fft.y_pass_t<T>((void* const*)sp, st, V, +1, 2, c->s));
+        int nblk = 0;
+        K.xclamp(true, c->s, c->dp, stg, nc, phi, (const T*)c->cons, n3, c->fft.twLx, c->fft.twNx, c->fft.nx,
+                 c->fft.ny, c->fft.nz, c->part, c->gate, &nblk);
+        MM_HIP(hipGetLastError());
+        return finalize_clamp(c, nblk);
+      }
       MM_TRY(inv_yx<T>(c, sp, rp, V));
     } else {
       MM_TRY(fwd<T>(c, (T*)c->fdiff, n3, work, V));
@@ -1668,14 +1810,7 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   if (rec_phipre) MM_HIP(hipMemcpyAsync(rec_phipre, phi, (size_t)V * n3 * sizeof(T), hipMemcpyDeviceToDevice, c->s));
   K.clamp(gp, c->s, c->dp, phi, (T*)c->cons, c->part, n3, c->gate);
   MM_HIP(hipGetLastError());
-  FinalizeOps fo;
-  for (int e = 0; e < E; ++e) fo.op[e] = 0;
-  fo.op[E] = 1; fo.op[E + 1] = 2; fo.op[E + 2] = 2; fo.op[E + 3] = 0;
-  for (int i = E + 4; i < 8; ++i) fo.op[i] = 0;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, c->s, c->part, c->nblk_pw, E + 4, fo, c->dres + R_CLAMP,
-                     c->gate);
-  MM_HIP(hipGetLastError());
-  return 0;
+  return finalize_clamp(c, c->nblk_pw);
 }
 
 // fill the fort.98/99 row from the result slots r, advance `step`
@@ -2018,6 +2153,14 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
                     c->k32.xelener(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx, p->ny,
                                    p->nz, nullptr, nullptr, nullptr, nullptr, nullptr, &nb);
       if (nb > kMaxPartBlocks) c->xpolar = false;
+      nb = 0;
+      if (precision == MM_FP64)
+        c->xclamp = c->k64.xclamp(false, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, p->nx,
+                                  p->ny, p->nz, nullptr, nullptr, &nb);
+      else
+        c->xclamp = c->k32.xclamp(false, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, p->nx,
+                                  p->ny, p->nz, nullptr, nullptr, &nb);
+      if (nb > kMaxPartBlocks || c->V > kStageN) c->xclamp = false;
     }
     const char* zv = std::getenv("MM_ZF_ELASTIC_VARIANT");
     if (zv) c->zf_el_var = std::max(0, std::min(kZfElasticVariants - 1, std::atoi(zv)));
