@@ -128,6 +128,21 @@ int mm_step(mm_ctx *ctx, int32_t nsteps, mm_diag *diag);
 int mm_get_step(const mm_ctx *ctx);
 int mm_set_step(mm_ctx *ctx, int32_t step);
 
+/* ---- Langevin noise (Multi_MEUMAPPS.f:552-571, 963-978) ----
+ * Enable the noise of the steps with step <= noise_step: every such step
+ * adds one langevin(iseed, sd) draw per point and variant (mm_langevin
+ * below), in the reference's order (z, y, x outer to inner, variants
+ * innermost), to the accumulated field phi_noise, whose transform enters the
+ * Allen-Cahn update (+ t_step * F(phi_noise), :973); later steps reset
+ * phi_noise to 0 (:569).  (idum, state) is the ran_2 generator at the first
+ * noise step, as for mm_ran2: for a restart run idum = iseed - myid (Input.txt
+ * `iseed`, :140, :229) and state = NULL (program-start state).  The draws are
+ * generated on the host (the stream is sequential).  Without this call a
+ * step with step <= noise_step returns MM_ERR_ARG.  Needs b_mob != 0.
+ * mm_get_noise_state returns the generator after the last noise step. */
+int mm_set_noise(mm_ctx *ctx, double sd, int32_t idum, const int32_t *state);
+int mm_get_noise_state(const mm_ctx *ctx, int32_t *idum, int32_t *state);
+
 /* Graph mode (no reference counterpart): the regular step (step > elastep0)
  * is recorded once into a HIP graph and replayed, with the elastic
  * convergence control of ModElastic.f:435-481 running on the device over
@@ -250,6 +265,9 @@ size_t mm_fft_spectrum_elems(const mm_fft *f);
 /* Numerical-Recipes ran2 as in ran_2.f:1-42, including the float32-folded
  * constants; state is kept in *state (35 int32) instead of SAVE storage. */
 double mm_ran2(int32_t *idum, int32_t *state);
+/* langevin(iseed, sd, harvest) (ModFunc.f:77-100): polar Box-Muller on two
+ * ran2 draws per attempt; returns the Gaussian number times sd. */
+double mm_langevin(int32_t *idum, int32_t *state, double sd);
 /* Uniform initial fields c = cav + a*(2r-1), phi_v = a*(2r-1) drawn from one
  * ran2 stream in Fortran order (x fastest; per cell: c_1..c_E then phi_1..V). */
 int mm_ic_uniform(int32_t nx, int32_t ny, int32_t nz, int32_t nvar, int32_t nel,

@@ -64,6 +64,15 @@ __global__ void __launch_bounds__(kBlock) k_fill(T* __restrict__ p, size_t n, T 
   if (gated_off(gate)) return;
   for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) p[i] = v;
 }
+// Allen-Cahn source with Langevin noise (Multi_MEUMAPPS.f:963-973):
+// F(phi) - dt b F(f_diff) + dt F(phi_noise) = F(phi) - dt b F(f_diff - phi_noise / b),
+// so the noise joins f_diff before the one forward transform
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_noise_fold(const T* __restrict__ fdiff, const T* __restrict__ noise,
+                                                        T* __restrict__ out, size_t n, double b_mob) {
+  for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    out[i] = (T)((double)fdiff[i] - (double)noise[i] / b_mob);
+}
 __global__ void k_ctl_start(Ctl* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctl_start(*c);
 }
@@ -1335,6 +1344,17 @@ struct mm_ctx {
   // the Allen-Cahn inverse x pass runs fused with the clamp (needs zfused and a
   // fused kernel for Nx; MM_XFUSED=0 switches it off)
   bool xclamp = false;
+  // Langevin noise (steps with step <= noise_step): generator state, the
+  // accumulated phi_noise (host, binary64, SoA), its device copy and a pinned
+  // staging buffer in the storage type
+  bool noise_on = false;
+  bool noise_live = false;  // phi_noise is nonzero (a noise step ran since the last reset)
+  double noise_sd = 0.0;
+  int32_t rng_idum = 0;      // ran_2's idum and its saved state (mm_ran2)
+  int32_t rng_state[35] = {};
+  std::vector<double> noise_acc;
+  void* noise_dev = nullptr;
+  void* noise_pin = nullptr;
   int last_niter = 0;
   double last_cfac = 1.0;
   // graph mode: the regular step (step > elastep0) recorded once and replayed
@@ -1775,10 +1795,18 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   {
     C2<T>* sp[MM_MAX_VAR];
     T* rp[MM_MAX_VAR];
+    T* fsrc = (T*)c->fdiff;
+    if (c->step <= c->prm.noise_step) {
+      // a noise step: phi_noise (already on the device) joins f_diff
+      hipLaunchKernelGGL(k_noise_fold<T>, gp, dim3(kBlock), 0, c->s, (const T*)c->fdiff, (const T*)c->noise_dev,
+                         (T*)c->xpol, (size_t)V * n3, c->prm.b_mob);
+      MM_HIP(hipGetLastError());
+      fsrc = (T*)c->xpol;
+    }
     if (c->zfused) {
       const T* fin[MM_MAX_VAR];
       C2<T>* fout[MM_MAX_VAR];
-      for (int v = 0; v < V; ++v) { fin[v] = (T*)c->fdiff + (size_t)v * n3; fout[v] = work + (size_t)v * nc; }
+      for (int v = 0; v < V; ++v) { fin[v] = fsrc + (size_t)v * n3; fout[v] = work + (size_t)v * nc; }
       MM_TRY(fwd_xy<T>(c, fin, fout, V));
       OpAC<T> opac{c->dp, work, sphi, nc};
       opac.gate = c->gate;
@@ -1799,7 +1827,7 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
       }
       MM_TRY(inv_yx<T>(c, sp, rp, V));
     } else {
-      MM_TRY(fwd<T>(c, (T*)c->fdiff, n3, work, V));
+      MM_TRY(fwd<T>(c, fsrc, n3, work, V));
       hipLaunchKernelGGL(k_ks_ac<T>, dim3(c->nblk_k, V), dim3(kBlock), 0, c->s, c->dp, sphi, work, nc);
       MM_HIP(hipGetLastError());
       for (int v = 0; v < V; ++v) { sp[v] = sphi + (size_t)v * nc; rp[v] = phi + (size_t)v * n3; }
@@ -1855,10 +1883,32 @@ int step_collect(mm_ctx* c, mm_diag* d) {
 }
 
 // eager step: host-driven elastic control
+// Multi_MEUMAPPS.f:552-571: this step's draws added to phi_noise on the host
+// (binary64, the reference's order), then copied to the device
+template <typename T> int noise_advance(mm_ctx* c) {
+  const size_t n3 = c->n3;
+  const int V = c->V;
+  double* acc = c->noise_acc.data();
+  noise_add(&c->rng_idum, c->rng_state, c->noise_sd, V, n3, acc);
+  T* pin = (T*)c->noise_pin;
+  for (size_t i = 0; i < (size_t)V * n3; ++i) pin[i] = (T)acc[i];
+  MM_HIP(hipMemcpyAsync(c->noise_dev, pin, (size_t)V * n3 * sizeof(T), hipMemcpyHostToDevice, c->s));
+  c->noise_live = true;
+  return 0;
+}
+
 template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
-  if (c->step <= c->prm.noise_step) return MM_ERR_ARG;  // Langevin noise is host-side, not supported here
+  const bool noisy = c->step <= c->prm.noise_step;
+  if (noisy && !c->noise_on) return MM_ERR_ARG;  // mm_set_noise first
+  if (!noisy && c->noise_live) {
+    // steps past noise_step: phi_noise = 0 (:569)
+    std::fill(c->noise_acc.begin(), c->noise_acc.end(), 0.0);
+    c->noise_live = false;
+  }
   set_gate(c, nullptr);
   MM_TRY(step_head<T>(c));
+  // the host draws overlap the head on the device
+  if (noisy) MM_TRY(noise_advance<T>(c));
   const bool elastic = c->step >= c->prm.elastep0;
   c->last_niter = 0;
   c->last_cfac = 1.0;
@@ -1957,6 +2007,8 @@ void free_all(mm_ctx* c) {
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
+  if (c->noise_dev) (void)hipFree(c->noise_dev);
+  if (c->noise_pin) (void)hipHostFree(c->noise_pin);
   if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->ev) (void)hipEventDestroy(c->ev);
   c->probe.destroy();
@@ -2249,6 +2301,31 @@ extern "C" int mm_get_step(const mm_ctx* c) { return c ? c->step : 0; }
 extern "C" int mm_set_step(mm_ctx* c, int32_t step) {
   if (!c) return MM_ERR_ARG;
   c->step = step;
+  return 0;
+}
+
+extern "C" int mm_set_noise(mm_ctx* c, double sd, int32_t idum, const int32_t* state) {
+  if (!c || c->prm.b_mob == 0.0) return MM_ERR_ARG;
+  const size_t bytes = (size_t)c->V * c->n3 * c->esz;
+  if (!c->noise_dev) MM_TRY(dalloc(&c->noise_dev, bytes));
+  if (!c->noise_pin && hipHostMalloc(&c->noise_pin, bytes, hipHostMallocDefault) != hipSuccess) {
+    c->noise_pin = nullptr;
+    return MM_ERR_NOMEM;
+  }
+  c->noise_acc.assign((size_t)c->V * c->n3, 0.0);
+  c->rng_idum = idum;
+  if (state) std::memcpy(c->rng_state, state, sizeof(c->rng_state));
+  else std::memset(c->rng_state, 0, sizeof(c->rng_state));
+  c->noise_sd = sd;
+  c->noise_on = true;
+  c->noise_live = false;
+  return 0;
+}
+
+extern "C" int mm_get_noise_state(const mm_ctx* c, int32_t* idum, int32_t* state) {
+  if (!c || !idum || !state) return MM_ERR_ARG;
+  *idum = c->rng_idum;
+  std::memcpy(state, c->rng_state, sizeof(c->rng_state));
   return 0;
 }
 

@@ -10,8 +10,9 @@ reaches a multiple of ifreq (:1069-1100).  The loop runs the body for
 step = step0 .. Nt - 1, as the reference's ``if (step .lt. Nt) go to 4``.
 
 Initial conditions other than a restart file (InitFields / NucAdd,
-ModInitials.f:415-826) and Langevin noise are host work outside this path:
-run() refuses nrun != 2 and noise_step >= step.
+ModInitials.f:415-826) are host work outside this path: run() refuses
+nrun != 2.  Noise steps (step <= noise_step, :552-571) run with the ran_2
+stream seeded by Input.txt's iseed (one process: myid = 0, :229).
 
     python -m meumapps_ss_amd.driver Input.txt data_save.0001 outdir [--mixed]
 """
@@ -32,8 +33,6 @@ def run(input_txt: str, restart: str, outdir: str, precision: int = MM_FP64, sym
     if inp.nrun != 2:
         raise NotImplementedError("only restart runs (nrun = 2): initial fields come from data_save.0001")
     p = setup(inp)
-    if p.noise_step >= p.step:
-        raise NotImplementedError("Langevin noise steps (step <= noise_step) are not run on the GPU")
     shape = (p.nx, p.ny, p.nz)
     V, E = p.vartot, p.numel
     rec = read_data_save(restart, V, E)
@@ -43,6 +42,8 @@ def run(input_txt: str, restart: str, outdir: str, precision: int = MM_FP64, sym
     sim = Simulation(p, precision)
     try:
         sim.upload_records(rec)
+        if p.noise_step >= p.step:
+            sim.set_noise(inp.sd, inp.iseed)
         if graph:
             sim.set_graph(True)
         step = p.step

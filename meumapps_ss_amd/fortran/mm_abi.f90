@@ -106,6 +106,38 @@ module mm_abi
        real(c_double), intent(out) :: phi(*), cons(*)
      end function mm_ic_uniform
 
+     ! ran_2.f:1-42 with its SAVE state in state(35) (all zero at program start)
+     real(c_double) function mm_ran2(idum, state) bind(c, name="mm_ran2")
+       import :: c_double, c_int32_t
+       integer(c_int32_t), intent(inout) :: idum
+       integer(c_int32_t), intent(inout) :: state(35)
+     end function mm_ran2
+
+     ! ModFunc.f:77-100 on the ran_2 stream (idum, state(35))
+     real(c_double) function mm_langevin(idum, state, sd) bind(c, name="mm_langevin")
+       import :: c_double, c_int32_t
+       integer(c_int32_t), intent(inout) :: idum
+       integer(c_int32_t), intent(inout) :: state(35)
+       real(c_double), value :: sd
+     end function mm_langevin
+
+     ! noise of the steps step <= noise_step: a restart run passes
+     ! idum = iseed - myid and c_null_ptr for the program-start state
+     integer(c_int) function mm_set_noise(ctx, sd, idum, state) bind(c, name="mm_set_noise")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       real(c_double), value :: sd
+       integer(c_int32_t), value :: idum
+       type(c_ptr), value :: state
+     end function mm_set_noise
+
+     integer(c_int) function mm_get_noise_state(ctx, idum, state) bind(c, name="mm_get_noise_state")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: idum
+       integer(c_int32_t), intent(out) :: state(35)
+     end function mm_get_noise_state
+
      integer(c_int) function mm_set_graph(ctx, on, iters) bind(c, name="mm_set_graph")
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx

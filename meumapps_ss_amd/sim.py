@@ -57,6 +57,20 @@ class Simulation:
         """Replay the regular step as one recorded graph (results identical)."""
         _lib.check(self.lib.mm_set_graph(self.ctx, int(on), iters), "mm_set_graph")
 
+    def set_noise(self, sd: float, idum: int, state=None):
+        """Langevin noise of the steps with step <= noise_step (mm_set_noise):
+        amplitude sd, ran_2 generator (idum, 35-word state; None = program
+        start).  For a restart run idum = Input.txt iseed (minus the rank)."""
+        st = None if state is None else (ctypes.c_int32 * 35)(*state)
+        _lib.check(self.lib.mm_set_noise(self.ctx, float(sd), int(idum), st), "mm_set_noise")
+
+    def noise_state(self):
+        """(idum, 35-word state) of the generator after the last noise step."""
+        idum = ctypes.c_int32(0)
+        st = (ctypes.c_int32 * 35)()
+        _lib.check(self.lib.mm_get_noise_state(self.ctx, ctypes.byref(idum), st), "mm_get_noise_state")
+        return idum.value, list(st)
+
     def __enter__(self):
         return self
 

@@ -19,13 +19,15 @@
  *   Cahn-Hilliard update ............ Multi_MEUMAPPS.f:910-949
  *   Allen-Cahn update ............... Multi_MEUMAPPS.f:955-1001
  *   clamp + check loop .............. Multi_MEUMAPPS.f:1008-1065
+ *   Langevin noise .................. Multi_MEUMAPPS.f:552-571, 963-978 with
+ *                                    ran_2.f:1-43 and langevin ModFunc.f:77-100
  *
  * Deviations that only move bits in the last place: the FFT is this file's
  * own mixed-radix Cooley-Tukey (not FFTW); the LINPACK rcond estimate is
  * replaced by the exact 1-norm condition number of the computed inverse;
  * global sums are taken per z-slab and then over slabs in z order, which is
- * deterministic for any thread count.  Langevin noise (step <= noise_step)
- * is not restated (MM_ERR_ARG).
+ * deterministic for any thread count.  A noise step (step <= noise_step)
+ * needs oracle_set_noise first (MM_ERR_ARG otherwise).
  */
 #include "oracle.h"
 
@@ -234,6 +236,11 @@ struct oracle {
   double *qinv; /* 9 per k (column-major 3x3) */
   int qinv_ready;
   int step;
+  /* Langevin noise: ran_2 state (idum, then idum2, iy, iv(32), init flag) */
+  int noise_on;
+  double noise_sd;
+  int32_t r_idum, r_state[35];
+  double *phi_noise; /* V*N3, accumulated over the noise steps */
 };
 
 #define RIDX(o, x, y, z) ((long)(x) + (long)(o)->nx * ((long)(y) + (long)(o)->ny * (long)(z)))
@@ -294,8 +301,84 @@ void oracle_destroy(oracle_t *o) {
                     o->gu, o->el_en, o->gradx, o->grady, o->gradz, o->kdel, o->fgrad,
                     o->fdiff, o->elgrad, o->Pphi, o->Fx, o->Fy, o->Fz, o->xf, o->u_old, o->qinv};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) free(bufs[i]);
-  free(o->sphi); free(o->sw);
+  free(o->sphi); free(o->sw); free(o->phi_noise);
   free(o);
+}
+
+/* ran_2.f:1-43: L'Ecuyer's two multiplicative generators (Schrage's
+ * decomposition keeps 32-bit products) with a 32-entry Bays-Durham shuffle.
+ * st[0] = idum2, st[1] = iy, st[2..33] = iv(1..32); st[34] != 0 once the
+ * DATA values (idum2 = 123456789, iv = 0, iy = 0) are in place.  am and rnmx
+ * follow the Fortran literals: 1./im1 is a binary32 quotient. */
+double oracle_ran2(int32_t *idum, int32_t *st) {
+  const int32_t im1 = 2147483563, im2 = 2147483399, ia1 = 40014, ia2 = 40692, iq1 = 53668, iq2 = 52774,
+                ir1 = 12211, ir2 = 3791, ntab = 32, ndiv = 1 + (im1 - 1) / ntab;
+  if (!st[34]) {
+    st[0] = 123456789; st[1] = 0;
+    for (int j = 0; j < ntab; ++j) st[2 + j] = 0;
+    st[34] = 1;
+  }
+  int32_t *iv = st + 2, k;
+  if (*idum <= 0) {
+    *idum = -*idum > 1 ? -*idum : 1;
+    st[0] = *idum;
+    for (int j = ntab + 8; j >= 1; --j) {
+      k = *idum / iq1;
+      *idum = ia1 * (*idum - k * iq1) - k * ir1;
+      if (*idum < 0) *idum += im1;
+      if (j <= ntab) iv[j - 1] = *idum;
+    }
+    st[1] = iv[0];
+  }
+  k = *idum / iq1;
+  *idum = ia1 * (*idum - k * iq1) - k * ir1;
+  if (*idum < 0) *idum += im1;
+  k = st[0] / iq2;
+  st[0] = ia2 * (st[0] - k * iq2) - k * ir2;
+  if (st[0] < 0) st[0] += im2;
+  int32_t j = st[1] / ndiv;
+  st[1] = iv[j] - st[0];
+  iv[j] = *idum;
+  if (st[1] < 1) st[1] += im1 - 1;
+  const double am = (double)(1.0f / (float)im1), rnmx = 1.0 - 1.2e-7;
+  double r = am * (double)st[1];
+  return r < rnmx ? r : rnmx;
+}
+
+/* ModFunc.f:77-100 */
+double oracle_langevin(int32_t *idum, int32_t *st, double sd) {
+  for (;;) {
+    double v1 = oracle_ran2(idum, st), v2 = oracle_ran2(idum, st);
+    v1 = 2.0 * v1 - 1.0;
+    v2 = 2.0 * v2 - 1.0;
+    double a = v1 * v1, b = v2 * v2;
+    double rsq = a + b;
+    if (rsq > 0.0 && rsq < 1.0) {
+      double l = log(rsq);
+      rsq = sqrt(-2.0 * l / rsq);
+      double h = v2 * rsq;
+      return h * sd;
+    }
+  }
+}
+
+int oracle_set_noise(oracle_t *o, double sd, int32_t idum, const int32_t *state) {
+  if (!o->phi_noise) o->phi_noise = xcalloc(o->V * o->n3, 8);
+  if (!o->phi_noise) return MM_ERR_NOMEM;
+  memset(o->phi_noise, 0, sizeof(double) * o->V * o->n3);
+  o->r_idum = idum;
+  if (state) memcpy(o->r_state, state, sizeof(o->r_state));
+  else memset(o->r_state, 0, sizeof(o->r_state));
+  o->noise_sd = sd;
+  o->noise_on = 1;
+  return 0;
+}
+
+int oracle_get_noise(oracle_t *o, int32_t *idum, int32_t *state, double *phi_noise) {
+  if (idum) *idum = o->r_idum;
+  if (state) memcpy(state, o->r_state, sizeof(o->r_state));
+  if (phi_noise && o->phi_noise) memcpy(phi_noise, o->phi_noise, sizeof(double) * o->V * o->n3);
+  return 0;
 }
 
 int oracle_set_threads(oracle_t *o, int nthreads) {
@@ -899,7 +982,18 @@ int oracle_step(oracle_t *o, mm_diag *d) {
   const int V = o->V, E = o->E, H = o->H, ny = o->ny, nz = o->nz, nt = o->nthreads;
   long n3 = o->n3, nc = o->nc;
   int rc;
-  if (o->step <= o->p.noise_step) return MM_ERR_ARG; /* Langevin noise not restated */
+  const int noisy = o->step <= o->p.noise_step;
+  if (noisy && !o->noise_on) return MM_ERR_ARG;
+  /* noise, :552-571: one langevin draw per point and variant, z/y/x/variant order */
+  if (noisy) {
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < o->nx; ++x)
+          for (int v = 0; v < V; ++v)
+            o->phi_noise[v * n3 + RIDX(o, x, y, z)] += oracle_langevin(&o->r_idum, o->r_state, o->noise_sd);
+  } else if (o->phi_noise) {
+    memset(o->phi_noise, 0, sizeof(double) * V * n3);
+  }
   mm_diag dd;
   memset(&dd, 0, sizeof(dd));
   dd.step = o->step;
@@ -949,18 +1043,20 @@ int oracle_step(oracle_t *o, mm_diag *d) {
     }
     inv_trans(o, sc, o->cons + e * n3);
   }
-  /* Allen-Cahn, :955-1001 (noise term is zero: step > noise_step) */
-  cplx *sf = o->sw + nc;
+  /* Allen-Cahn, :955-1001 (term3 = t_step F(phi_noise), zero past noise_step) */
+  cplx *sf = o->sw + nc, *sn = o->sw + 2 * nc;
   for (int v = 0; v < V; ++v) {
     cplx *sp = o->sphi + v * nc; /* FFT of phi from the gradient block; phi unchanged since */
     f_trans(o, o->fdiff + v * n3, sf);
-    double dtl = o->p.t_step * o->p.b_mob;
+    if (noisy) f_trans(o, o->phi_noise + v * n3, sn);
+    double dtl = o->p.t_step * o->p.b_mob, dt = o->p.t_step;
 #pragma omp parallel for num_threads(nt) schedule(static)
     for (long idx = 0; idx < nc; ++idx) {
       int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
       cplx t1 = dtl * sf[idx];
       double t2 = 1.0 - dtl * kgk(o, v, kx, ky, kz);
       cplx num = sp[idx] - t1;
+      if (noisy) num = num + sn[idx] * dt;
       sp[idx] = creal(num) / t2 + I * (cimag(num) / t2);
     }
     inv_trans(o, sp, o->phi + v * n3);

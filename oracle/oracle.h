@@ -52,6 +52,17 @@ int oracle_invert(double *a /* n*n column-major */, int n, int flag);
  * kf_im[m] = imaginary part of kf, kq[m] = kf_sq (real, negative) */
 void oracle_kaxis(int n, int count, double *kf_im, double *kq);
 
+/* ran_2.f:1-43 and langevin (ModFunc.f:77-100); st holds 35 int32 (idum2,
+ * iy, iv(32), init flag), all zero at program start */
+double oracle_ran2(int32_t *idum, int32_t *st);
+double oracle_langevin(int32_t *idum, int32_t *st, double sd);
+/* noise of the steps step <= noise_step (Multi_MEUMAPPS.f:552-571, 963-978)
+ * from the generator (idum, state; state NULL = program start) */
+int oracle_set_noise(oracle_t *o, double sd, int32_t idum, const int32_t *state);
+/* generator after the last noise step and the accumulated phi_noise (V*N3);
+ * any pointer may be NULL */
+int oracle_get_noise(oracle_t *o, int32_t *idum, int32_t *state, double *phi_noise);
+
 /* elastic solve alone (eldis + elener) on the current state, given P_phi
  * fields computed from phi; returns 0 or error and the iteration count */
 int oracle_eldis(oracle_t *o, int *n_iter, double *cfac);

@@ -49,6 +49,10 @@ def load():
         "oracle_invert": (c_int, [D, c_int, c_int]),
         "oracle_kaxis": (None, [c_int, c_int, D, D]),
         "oracle_eldis": (c_int, [c_void_p, POINTER(c_int), D]),
+        "oracle_ran2": (c_double, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32)]),
+        "oracle_langevin": (c_double, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), c_double]),
+        "oracle_set_noise": (c_int, [c_void_p, c_double, ctypes.c_int32, POINTER(ctypes.c_int32)]),
+        "oracle_get_noise": (c_int, [c_void_p, POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), D]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -134,6 +138,19 @@ class Oracle:
         out = np.empty(self.shape)
         self.lib.oracle_inv_trans(self.h, s.ctypes.data_as(POINTER(c_double)), _p(out))
         return out
+
+    def set_noise(self, sd, idum, state=None):
+        st = None if state is None else (ctypes.c_int32 * 35)(*state)
+        rc = self.lib.oracle_set_noise(self.h, sd, idum, st)
+        if rc:
+            raise RuntimeError(f"oracle_set_noise error {rc}")
+
+    def get_noise(self):
+        idum = ctypes.c_int32(0)
+        st = (ctypes.c_int32 * 35)()
+        pn = np.empty((self.V, *self.shape))
+        self.lib.oracle_get_noise(self.h, ctypes.byref(idum), st, _p(pn))
+        return idum.value, list(st), pn
 
     def eldis(self):
         n = c_int(0)
