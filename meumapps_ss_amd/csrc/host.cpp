@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <thread>
+#include <vector>
 #include <cstdio>
 #include <cstring>
 
@@ -131,13 +134,130 @@ extern "C" double mm_langevin(int32_t* idum, int32_t* state, double sd) {
   return r;
 }
 
+namespace {
+
+// x * a^k mod m (exact in 64-bit: m < 2^31)
+int64_t mod_pow_mul(int64_t x, int64_t a, uint64_t k, int64_t m) {
+  int64_t r = 1 % m, b = a % m;
+  while (k) {
+    if (k & 1) r = (r * b) % m;
+    b = (b * b) % m;
+    k >>= 1;
+  }
+  return (x * r) % m;
+}
+
+int noise_threads() {
+  unsigned hw = std::thread::hardware_concurrency();
+  int t = hw ? (int)hw : 1;
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) t = std::min(t, v);
+  }
+  return std::max(1, std::min(t, 16));
+}
+
+template <class F> void parallel_for(int nt, size_t n, F&& f) {
+  if (nt <= 1 || n < 4096) { f(0, n); return; }
+  std::vector<std::thread> th;
+  const size_t step = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const size_t lo = std::min(n, (size_t)t * step), hi = std::min(n, lo + step);
+    if (lo < hi) th.emplace_back([&f, lo, hi] { f(lo, hi); });
+  }
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
 namespace mm {
 // Multi_MEUMAPPS.f:554-562 for one process: points in linear (z, y, x) order,
-// variants innermost; acc is SoA (V fields of n3)
+// variants innermost; acc is SoA (V fields of n3).
+//
+// The draws are those of mm_langevin called V*n3 times in a row, computed in
+// blocks: the two multiplicative streams of ran_2 (idum, idum2) are
+// independent of the shuffle, so threads produce them from jump-ahead
+// starting points (x a^k mod m, exact in 64-bit, the same integers as the
+// Schrage steps); one pass in order runs the shuffle table, forms the
+// uniforms and the polar pairs and keeps the accepted ones, stopping at the
+// last draw needed; threads then evaluate v2 sqrt(-2 ln rsq / rsq) sd with the
+// scalar libm calls.  The generator state left behind is the sequential one.
 void noise_add(int32_t* idum, int32_t* state, double sd, int V, size_t n3, double* acc) {
   Ran2 g(idum, state);
-  for (size_t i = 0; i < n3; ++i)
-    for (int v = 0; v < V; ++v) acc[(size_t)v * n3 + i] += g.langevin(sd);
+  const size_t N = (size_t)V * n3;
+  const int nt = noise_threads();
+  if (N < ((size_t)1 << 16) || nt == 1) {
+    for (size_t i = 0; i < n3; ++i)
+      for (int v = 0; v < V; ++v) acc[(size_t)v * n3 + i] += g.langevin(sd);
+    g.store(idum, state);
+    return;
+  }
+  const int64_t m1 = 2147483563, m2 = 2147483399, a1 = 40014, a2 = 40692;
+  const int32_t imm1 = (int32_t)(m1 - 1), ndiv = 1 + imm1 / 32;
+  const double am = (double)(1.0f / (float)m1), rnmx = 1.0 - 1.2e-7;
+  size_t done = 0;  // draws produced so far, in (point, variant) order
+  if (g.idum <= 0) {
+    // the first draw initialises the table: take it on the scalar path
+    acc[0] += g.langevin(sd);
+    done = 1;
+  }
+  const size_t U = (size_t)1 << 22;  // uniforms per block (even)
+  std::vector<int32_t> s1(U), s2(U);
+  std::vector<double> pv2(U / 2 + 1), prsq(U / 2 + 1);
+  while (done < N) {
+    // 1. the two streams for the next U uniforms
+    const int64_t x1 = g.idum, x2 = g.idum2;
+    parallel_for(nt, U, [&](size_t lo, size_t hi) {
+      int64_t y1 = mod_pow_mul(x1, a1, lo + 1, m1), y2 = mod_pow_mul(x2, a2, lo + 1, m2);
+      for (size_t n = lo; n < hi; ++n) {
+        s1[n] = (int32_t)y1;
+        s2[n] = (int32_t)y2;
+        y1 = (y1 * a1) % m1;
+        y2 = (y2 * a2) % m2;
+      }
+    });
+    // 2. shuffle, uniforms, polar pairs, in order; stop after the last needed draw
+    // (branch-free: the sign test of iy and the acceptance test are data
+    // dependent and would mispredict about half and a fifth of the time)
+    const size_t need = N - done;
+    size_t used = 0, cnt = 0;
+    int32_t iy = g.iy;
+    while (used + 2 <= U && cnt < need) {
+      double u[2];
+      for (int h = 0; h < 2; ++h) {
+        const size_t n = used + h;
+        const int32_t j = iy / ndiv;
+        iy = g.iv[j] - s2[n];
+        g.iv[j] = s1[n];
+        iy += iy < 1 ? imm1 : 0;
+        const double x = am * (double)iy;
+        u[h] = x < rnmx ? x : rnmx;
+      }
+      used += 2;
+      const double v1 = 2.0 * u[0] - 1.0, v2 = 2.0 * u[1] - 1.0;
+      const double v1s = v1 * v1, v2s = v2 * v2;
+      const double rsq = v1s + v2s;
+      pv2[cnt] = v2;
+      prsq[cnt] = rsq;
+      cnt += (rsq > 0.0) & (rsq < 1.0);
+    }
+    g.iy = iy;
+    g.idum = s1[used - 1];
+    g.idum2 = s2[used - 1];
+    // 3. the Gaussian numbers, added in (point, variant) order
+    const size_t base = done;
+    parallel_for(nt, cnt, [&](size_t lo, size_t hi) {
+      for (size_t q = lo; q < hi; ++q) {
+        const double lg = std::log(prsq[q]);
+        const double r = std::sqrt(-2.0 * lg / prsq[q]);
+        const double h = pv2[q] * r;
+        const size_t k = base + q, i = k / (size_t)V;
+        const int v = (int)(k % (size_t)V);
+        acc[(size_t)v * n3 + i] += h * sd;
+      }
+    });
+    done += cnt;
+  }
   g.store(idum, state);
 }
 }  // namespace mm
