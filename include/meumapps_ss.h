@@ -268,6 +268,13 @@ double mm_ran2(int32_t *idum, int32_t *state);
 /* langevin(iseed, sd, harvest) (ModFunc.f:77-100): polar Box-Muller on two
  * ran2 draws per attempt; returns the Gaussian number times sd. */
 double mm_langevin(int32_t *idum, int32_t *state, double sd);
+/* The noise loop of one step for one process (Multi_MEUMAPPS.f:554-562):
+ * phi_noise[v*n3 + i] += mm_langevin(...) for the points i in linear
+ * (z, y, x) order with the variants innermost; the draws and the generator
+ * left behind are those of the sequential calls (threads are used where the
+ * stream allows it). */
+int mm_noise_accumulate(int32_t *idum, int32_t *state, double sd, int32_t nvar, int64_t n3,
+                        double *phi_noise);
 /* Uniform initial fields c = cav + a*(2r-1), phi_v = a*(2r-1) drawn from one
  * ran2 stream in Fortran order (x fastest; per cell: c_1..c_E then phi_1..V). */
 int mm_ic_uniform(int32_t nx, int32_t ny, int32_t nz, int32_t nvar, int32_t nel,

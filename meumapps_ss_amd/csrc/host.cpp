@@ -262,6 +262,13 @@ void noise_add(int32_t* idum, int32_t* state, double sd, int V, size_t n3, doubl
 }
 }  // namespace mm
 
+extern "C" int mm_noise_accumulate(int32_t* idum, int32_t* state, double sd, int32_t nvar, int64_t n3,
+                                   double* phi_noise) {
+  if (!idum || !state || !phi_noise || nvar < 1 || n3 < 1) return MM_ERR_ARG;
+  mm::noise_add(idum, state, sd, nvar, (size_t)n3, phi_noise);
+  return 0;
+}
+
 extern "C" int mm_ic_uniform(int32_t nx, int32_t ny, int32_t nz, int32_t nvar, int32_t nel,
                              int32_t iseed, double amp, const double* cav, double* phi,
                              double* cons) {

@@ -121,6 +121,18 @@ module mm_abi
        real(c_double), value :: sd
      end function mm_langevin
 
+     ! the noise loop of one step for one process (Multi_MEUMAPPS.f:554-562)
+     integer(c_int) function mm_noise_accumulate(idum, state, sd, nvar, n3, phi_noise) &
+          bind(c, name="mm_noise_accumulate")
+       import :: c_int, c_double, c_int32_t, c_int64_t
+       integer(c_int32_t), intent(inout) :: idum
+       integer(c_int32_t), intent(inout) :: state(35)
+       real(c_double), value :: sd
+       integer(c_int32_t), value :: nvar
+       integer(c_int64_t), value :: n3
+       real(c_double), intent(inout) :: phi_noise(*)
+     end function mm_noise_accumulate
+
      ! noise of the steps step <= noise_step: a restart run passes
      ! idum = iseed - myid and c_null_ptr for the program-start state
      integer(c_int) function mm_set_noise(ctx, sd, idum, state) bind(c, name="mm_set_noise")

@@ -362,6 +362,12 @@ double oracle_langevin(int32_t *idum, int32_t *st, double sd) {
   }
 }
 
+/* Multi_MEUMAPPS.f:554-562 for one process, sequentially */
+void oracle_noise_accumulate(int32_t *idum, int32_t *st, double sd, int nvar, long n3, double *acc) {
+  for (long i = 0; i < n3; ++i)
+    for (int v = 0; v < nvar; ++v) acc[v * n3 + i] += oracle_langevin(idum, st, sd);
+}
+
 int oracle_set_noise(oracle_t *o, double sd, int32_t idum, const int32_t *state) {
   if (!o->phi_noise) o->phi_noise = xcalloc(o->V * o->n3, 8);
   if (!o->phi_noise) return MM_ERR_NOMEM;

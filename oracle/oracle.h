@@ -56,6 +56,8 @@ void oracle_kaxis(int n, int count, double *kf_im, double *kq);
  * iy, iv(32), init flag), all zero at program start */
 double oracle_ran2(int32_t *idum, int32_t *st);
 double oracle_langevin(int32_t *idum, int32_t *st, double sd);
+/* the noise loop of one step, sequentially: acc[v*n3 + i] += langevin */
+void oracle_noise_accumulate(int32_t *idum, int32_t *st, double sd, int nvar, long n3, double *acc);
 /* noise of the steps step <= noise_step (Multi_MEUMAPPS.f:552-571, 963-978)
  * from the generator (idum, state; state NULL = program start) */
 int oracle_set_noise(oracle_t *o, double sd, int32_t idum, const int32_t *state);

@@ -52,6 +52,8 @@ def load():
         "oracle_ran2": (c_double, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32)]),
         "oracle_langevin": (c_double, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), c_double]),
         "oracle_set_noise": (c_int, [c_void_p, c_double, ctypes.c_int32, POINTER(ctypes.c_int32)]),
+        "oracle_noise_accumulate": (None, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), c_double, c_int,
+                                           ctypes.c_long, D]),
         "oracle_get_noise": (c_int, [c_void_p, POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), D]),
     }
     for k, (r, a) in sig.items():

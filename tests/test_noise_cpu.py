@@ -107,3 +107,21 @@ def test_noise_step_without_generator_is_refused():
     o.set_state(phi=np.zeros((1, 8, 8, 8)), cons=np.full((1, 8, 8, 8), 0.4))
     with pytest.raises(RuntimeError):
         o.step(1)
+
+
+@pytest.mark.parametrize("seed,V,n3", [(-7, 3, 1500000), (12345, 1, 70000), (-2, 2, 40000)])
+def test_threaded_noise_loop_matches_sequential(seed, V, n3):
+    """mm_noise_accumulate (jump-ahead streams, threaded Box-Muller around one
+    in-order shuffle pass) against the oracle's sequential loop: the same
+    draws and generator over two consecutive steps; 1.5e6 points x 3 spans
+    several blocks of uniforms."""
+    lib, olib = _lib.load(), pyoracle.load()
+    D = ctypes.POINTER(ctypes.c_double)
+    ia, ib = ctypes.c_int32(seed), ctypes.c_int32(seed)
+    sa, sb = (ctypes.c_int32 * 35)(), (ctypes.c_int32 * 35)()
+    a, b = np.zeros(V * n3), np.zeros(V * n3)
+    for _ in range(2):
+        assert lib.mm_noise_accumulate(ctypes.byref(ia), sa, 0.2, V, n3, a.ctypes.data_as(D)) == 0
+        olib.oracle_noise_accumulate(ctypes.byref(ib), sb, 0.2, V, n3, b.ctypes.data_as(D))
+    assert np.array_equal(a, b)
+    assert ia.value == ib.value and list(sa) == list(sb)
