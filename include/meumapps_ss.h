@@ -265,6 +265,9 @@ size_t mm_fft_spectrum_elems(const mm_fft *f);
 /* Numerical-Recipes ran2 as in ran_2.f:1-42, including the float32-folded
  * constants; state is kept in *state (35 int32) instead of SAVE storage. */
 double mm_ran2(int32_t *idum, int32_t *state);
+/* n consecutive ran2 draws into out (the initial-field loops of
+ * ModInitials.f:415-690 draw one per point). */
+int mm_ran2_fill(int32_t *idum, int32_t *state, int64_t n, double *out);
 /* langevin(iseed, sd, harvest) (ModFunc.f:77-100): polar Box-Muller on two
  * ran2 draws per attempt; returns the Gaussian number times sd. */
 double mm_langevin(int32_t *idum, int32_t *state, double sd);

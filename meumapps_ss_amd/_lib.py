@@ -51,6 +51,7 @@ def load() -> ctypes.CDLL:
         "mm_fft_inverse": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
         "mm_fft_spectrum_elems": (c_size_t, [c_void_p]),
         "mm_ran2": (c_double, [POINTER(c_int32), POINTER(c_int32)]),
+        "mm_ran2_fill": (c_int32, [POINTER(c_int32), POINTER(c_int32), ctypes.c_int64, D]),
         "mm_langevin": (c_double, [POINTER(c_int32), POINTER(c_int32), c_double]),
         "mm_noise_accumulate": (c_int32, [POINTER(c_int32), POINTER(c_int32), c_double, c_int32,
                                           ctypes.c_int64, D]),
@@ -98,5 +99,5 @@ EXPORTED = [
     "mm_batch_create", "mm_batch_member", "mm_batch_step", "mm_batch_destroy",
     "mm_adj_enable", "mm_adj_zero", "mm_adj_set", "mm_adj_get", "mm_state_bytes", "mm_state_save",
     "mm_state_load", "mm_adj_tape_step", "mm_adj_step", "mm_langevin", "mm_set_noise",
-    "mm_get_noise_state", "mm_noise_accumulate",
+    "mm_get_noise_state", "mm_noise_accumulate", "mm_ran2_fill",
 ]

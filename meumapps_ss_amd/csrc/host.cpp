@@ -127,6 +127,14 @@ extern "C" double mm_ran2(int32_t* idum, int32_t* state) {
   return r;
 }
 
+extern "C" int mm_ran2_fill(int32_t* idum, int32_t* state, int64_t n, double* out) {
+  if (!idum || !state || n < 0 || (n > 0 && !out)) return MM_ERR_ARG;
+  Ran2 g(idum, state);
+  for (int64_t i = 0; i < n; ++i) out[i] = g.next();
+  g.store(idum, state);
+  return 0;
+}
+
 extern "C" double mm_langevin(int32_t* idum, int32_t* state, double sd) {
   Ran2 g(idum, state);
   const double r = g.langevin(sd);

@@ -1,20 +1,22 @@
 """Host driver of a run: the reference main program around the C ABI.
 
-Mirrors Multi_MEUMAPPS.f for a restart run (nrun = 2): read Input.txt
-(:117-206) and set up (:218-448, params.setup), read the restart record set
-data_save.0001 (:466-481), write the initial dump step_{step-1} (:496-520),
+Mirrors Multi_MEUMAPPS.f: read Input.txt (:117-206) and set up (:218-448,
+params.setup); take the initial fields from InitFields for a fresh start
+(nrun = 1, :455-459) or read the restart record set data_save.0001 and add
+nuclei with NucAdd (nrun = 2, :466-487; initfields.py); write the initial
+dump step_{step-1} (:496-520),
 then advance the loop body (:548-1106) through mm_step in chunks that end at
 the dump steps, writing the fort.98 / fort.99 rows of every step and the dump
 step_NNNNNNN/{data_extract_mult.in, data_save.0001} whenever the step counter
 reaches a multiple of ifreq (:1069-1100).  The loop runs the body for
 step = step0 .. Nt - 1, as the reference's ``if (step .lt. Nt) go to 4``.
 
-Initial conditions other than a restart file (InitFields / NucAdd,
-ModInitials.f:415-826) are host work outside this path: run() refuses
-nrun != 2.  Noise steps (step <= noise_step, :552-571) run with the ran_2
-stream seeded by Input.txt's iseed (one process: myid = 0, :229).
+InitFields, NucAdd and the noise steps (step <= noise_step, :552-571) draw
+from one ran_2 stream seeded by Input.txt's iseed (one process: myid = 0,
+:229), in that order.
 
     python -m meumapps_ss_amd.driver Input.txt data_save.0001 outdir [--mixed]
+    (the restart file is ignored when nrun = 1)
 """
 from __future__ import annotations
 
@@ -22,28 +24,40 @@ import argparse
 import os
 from typing import Optional
 
+import numpy as np
+
+from .initfields import Ran2, init_fields, nuc_add
 from .params import MM_FP64, MM_MIXED, read_input_txt, setup
 from .records import dump_step, fort98_row, fort99_row, read_data_save
-from .sim import Simulation
+from .sim import Simulation, aos_to_soa, soa_to_aos
 
 
 def run(input_txt: str, restart: str, outdir: str, precision: int = MM_FP64, sym_var: Optional[str] = None,
         graph: bool = False) -> dict:
     inp = read_input_txt(input_txt, sym_var)
-    if inp.nrun != 2:
-        raise NotImplementedError("only restart runs (nrun = 2): initial fields come from data_save.0001")
     p = setup(inp)
     shape = (p.nx, p.ny, p.nz)
     V, E = p.vartot, p.numel
-    rec = read_data_save(restart, V, E)
-    if tuple(rec["shape"]) != shape:
-        raise ValueError(f"restart grid {rec['shape']} differs from Input.txt {shape}")
+    rng = Ran2(inp.iseed)
+    soa_shape = (p.nz, p.ny, p.nx)
+    if inp.nrun == 1:
+        phi, cons = init_fields(inp, rng)
+        n3 = p.nx * p.ny * p.nz
+        rec = dict(phi=soa_to_aos(phi).ravel(), cons=soa_to_aos(cons).ravel(), el_en=np.zeros(n3),
+                   u=np.zeros((3, n3)), gradu=np.zeros((9, n3)))
+    else:
+        rec = read_data_save(restart, V, E)
+        if tuple(rec["shape"]) != shape:
+            raise ValueError(f"restart grid {rec['shape']} differs from Input.txt {shape}")
+        if inp.nucstat != "none":
+            phi = nuc_add(inp, aos_to_soa(rec["phi"], V, soa_shape), rng)
+            rec["phi"] = soa_to_aos(phi).ravel()
     os.makedirs(outdir, exist_ok=True)
     sim = Simulation(p, precision)
     try:
         sim.upload_records(rec)
         if p.noise_step >= p.step:
-            sim.set_noise(inp.sd, inp.iseed)
+            sim.set_noise(inp.sd, *rng.snapshot())
         if graph:
             sim.set_graph(True)
         step = p.step

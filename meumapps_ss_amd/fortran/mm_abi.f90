@@ -113,6 +113,15 @@ module mm_abi
        integer(c_int32_t), intent(inout) :: state(35)
      end function mm_ran2
 
+     ! n consecutive ran_2 draws
+     integer(c_int) function mm_ran2_fill(idum, state, n, out) bind(c, name="mm_ran2_fill")
+       import :: c_int, c_double, c_int32_t, c_int64_t
+       integer(c_int32_t), intent(inout) :: idum
+       integer(c_int32_t), intent(inout) :: state(35)
+       integer(c_int64_t), value :: n
+       real(c_double), intent(out) :: out(*)
+     end function mm_ran2_fill
+
      ! ModFunc.f:77-100 on the ran_2 stream (idum, state(35))
      real(c_double) function mm_langevin(idum, state, sd) bind(c, name="mm_langevin")
        import :: c_double, c_int32_t

@@ -66,3 +66,46 @@ def test_driver_run_and_resume(gpu_lib, tmp_path):
         assert np.abs(d6b[k] - d6[k]).max() < 1e-12, k
     r2 = (out2 / "fort.98").read_text().splitlines()
     assert [int(r[:8]) for r in r2] == list(range(3, inp.nt))
+
+
+def test_driver_fresh_start_and_noise(gpu_lib, tmp_path):
+    """nrun = 1: InitFields ('multiple' nuclei, 'cppt' compositions), then
+    noise steps from the same ran_2 stream; the initial dump holds the
+    InitFields fields, and the run equals the oracle fed those fields with the
+    generator state InitFields left."""
+    from meumapps_ss_amd.initfields import Ran2, init_fields
+    from meumapps_ss_amd.params import read_input_txt
+    from meumapps_ss_amd.sim import aos_to_soa
+    from oracle import Oracle
+    n = 16
+    inp = model_inputs(n, 3, (0.02, -0.01, -0.01), 0.005, nsteps=5)
+    inp.nrun, inp.ifreq = 1, 100
+    inp.nucstat, inp.targv, inp.nmaxppt, inp.iseed = "multiple", 2, 3, -17
+    inp.radppt, inp.phi_init, inp.constat = (2, 2, 2), 0.0, "cppt"
+    inp.conc = [(1, 0.4, 0.55, 0.0)]
+    inp.noise_step, inp.sd = 2, 0.01
+    write_input_txt(inp, str(tmp_path / "Input.txt"))
+    out = tmp_path / "fresh"
+    info = run(str(tmp_path / "Input.txt"), "", str(out))
+    assert info["dumps"] == [0]
+    inp_r = read_input_txt(str(tmp_path / "Input.txt"))
+    p = setup(inp_r)
+    rng = Ran2(inp_r.iseed)
+    phi0, cons0 = init_fields(inp_r, rng)
+    d0 = read_data_save(str(out / "step_0000000" / "data_save.0001"), p.vartot, p.numel)
+    assert np.array_equal(aos_to_soa(d0["phi"], p.vartot, (n, n, n)), phi0)
+    assert np.array_equal(aos_to_soa(d0["cons"], p.numel, (n, n, n)), cons0)
+    o = Oracle(p, 4)
+    o.set_state(phi=phi0, cons=cons0)
+    o.set_noise(inp_r.sd, *rng.snapshot())
+    o.step(inp_r.nt - inp_r.step)
+    ro = o.get_state()
+    with Simulation(p, MM_FP64) as s:
+        s.upload(phi=phi0, cons=cons0, u=np.zeros((3, n, n, n)), gradu=np.zeros((9, n, n, n)))
+        s.set_noise(inp_r.sd, *rng.snapshot())
+        s.step(inp_r.nt - inp_r.step)
+        st = s.download()
+    for k in ("phi", "cons"):
+        assert np.abs(st[k] - ro[k]).max() < 1e-10, k
+    rows = (out / "fort.98").read_text().splitlines()
+    assert [int(r[:8]) for r in rows] == list(range(inp_r.step, inp_r.nt))
