@@ -1272,10 +1272,13 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
   }
   static Kern<T> table(bool homog) {
     Kern<T> k;
-    k.xclamp = xclamp;
     k.chem = chem; k.ppoly = ppoly; k.clamp = clamp; k.zeroth = zeroth;
-    k.xpolar = homog ? xpolar : nullptr;
-    k.xelener = homog ? xelener : nullptr;
+    // the fused x kernels keep V fields per point in registers: up to 4
+    if constexpr (V <= 4) {
+      k.xclamp = xclamp;
+      k.xpolar = homog ? xpolar : nullptr;
+      k.xelener = homog ? xelener : nullptr;
+    }
     k.polar = homog ? polar<true> : polar<false>;
     k.elener = homog ? elener<true> : elener<false>;
     return k;
@@ -1284,7 +1287,8 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
 
 // supported (V, NPH, E)
 #define MM_COMBOS(X) \
-  X(1, 2, 1) X(2, 2, 1) X(3, 2, 1) X(4, 2, 1) X(2, 3, 1) X(3, 3, 1) X(4, 3, 1) X(1, 2, 2) X(3, 2, 2)
+  X(1, 2, 1) X(2, 2, 1) X(3, 2, 1) X(4, 2, 1) X(2, 3, 1) X(3, 3, 1) X(4, 3, 1) X(1, 2, 2) X(3, 2, 2) \
+  X(12, 2, 1)
 
 template <typename T> static bool kern_table(int V, int NPH, int E, bool homog, Kern<T>& out) {
 #define X(v_, p_, e_) \
@@ -2207,10 +2211,12 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
       if (nb > kMaxPartBlocks) c->xpolar = false;
       nb = 0;
       if (precision == MM_FP64)
-        c->xclamp = c->k64.xclamp(false, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, p->nx,
+        c->xclamp = c->k64.xclamp &&
+                    c->k64.xclamp(false, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, p->nx,
                                   p->ny, p->nz, nullptr, nullptr, &nb);
       else
-        c->xclamp = c->k32.xclamp(false, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, p->nx,
+        c->xclamp = c->k32.xclamp &&
+                    c->k32.xclamp(false, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, p->nx,
                                   p->ny, p->nz, nullptr, nullptr, &nb);
       if (nb > kMaxPartBlocks || c->V > kStageN) c->xclamp = false;
     }

@@ -36,7 +36,19 @@ CASES = {
     "v1_e2": (16, 1, (0.01, 0.01, 0.01), 0.005, {"numel_in": 3}, "rich"),
     # 16 x 16 x 512: the length-512 z kernels (radix-4 plan of the fused elastic pass)
     "v3_z512": (16, 3, (0.02, -0.01, -0.01), 0.005, {"nz": 512}, "rich"),
+    # nv = 12 from sym_var rotations (InitTransGrad, ModInitials.f:200-340): an
+    # orthorhombic eigenstrain turned about z in steps of 15 degrees
+    "v12_rot": (16, 12, (0.02, -0.01, 0.005), 0.005, {"sym_var": "z15"}, "rich"),
 }
+
+
+def z_rotations(n=12, step_deg=15.0):
+    out = []
+    for k in range(n):
+        a = np.deg2rad(step_deg * k)
+        c, s = np.cos(a), np.sin(a)
+        out.append(np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]]))
+    return np.stack(out)
 
 
 def make_case(name, seed=0):
@@ -44,8 +56,11 @@ def make_case(name, seed=0):
     kw = dict(kw)
     numel_in = kw.pop("numel_in", 2)
     nz = kw.pop("nz", n)
+    sym = kw.pop("sym_var", None)
     inp = model_inputs(n, nvar, tstr, dt, **kw)
     inp.nz = nz
+    if sym == "z15":
+        inp.sym_var = z_rotations()
     if numel_in != 2:
         # a second composition: A, G0, c0 rows extend to 2*numel+1
         inp.numel_in = numel_in

@@ -54,7 +54,7 @@ def test_first_step_intermediates(gpu_lib, name):
 
 
 @pytest.mark.parametrize("name,nsteps", [("v3_tetra", 40), ("v3_inhomog", 20), ("v2_twophase", 30),
-                                         ("v3_noise", 60), ("v1_e2", 20)])
+                                         ("v3_noise", 60), ("v1_e2", 20), ("v12_rot", 10)])
 def test_multi_step(gpu_lib, name, nsteps):
     p, phi, cons = make_case(name, seed=1)
     o = Oracle(p, 4)
