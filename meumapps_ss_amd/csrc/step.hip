@@ -1096,6 +1096,9 @@ k_ks_ch(const DevParams* __restrict__ P, C2<T>* __restrict__ work, size_t nc) {
     const double den = 1.0 - dtm * ksq;
     const C2<T> s = ch[i];
     ch[i] = mk<T>((T)(((double)s.x + dtm * tr) / den), (T)(((double)s.y + dtm * ti) / den));
+    // MIXED stores c - cbar: its mean is zero, and the FP32 rounding the
+    // forward transform leaves in the k = 0 mode must not accumulate
+    if (sizeof(T) == 4 && i == 0) ch[i] = mk<T>(0, 0);
   }
 }
 

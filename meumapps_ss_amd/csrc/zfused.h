@@ -381,6 +381,9 @@ template <typename T> struct OpCH {
       const KVec k = kfull(P, ck, tl + TPL * q);
       const double den = 1.0 - dtm * (k.kq[0] + k.kq[1] + k.kq[2]);
       v[q] = mk<T>((T)((st.cr[q] + dtm * st.tr[q]) / den), (T)((st.ci[q] + dtm * st.ti[q]) / den));
+      // MIXED stores c - cbar: its mean is zero, and the FP32 rounding the
+      // forward transform leaves in the k = 0 mode must not accumulate
+      if (sizeof(T) == 4 && k.kx == 0 && k.ky == 0 && k.kz == 0) v[q] = mk<T>(0, 0);
     }
   }
   template <int NW, class S> __device__ void reduce(const S&) const {}

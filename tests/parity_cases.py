@@ -32,7 +32,12 @@ CASES = {
     "v2_twophase": (16, 1, (0.02, 0.01, 0.01), 0.005, {"numph": 3}, "rich"),
     "v3_inhomog": (16, 3, (0.02, 0.01, -0.01), 0.005,
                    {"cela2": {(1, 1): 300.0, (1, 2): 140.0, (4, 4): 120.0}}, "rich"),
+    # nv = 2 has no eigenstrain variants in the reference set-up (ModInitials.f:200-340
+    # fills nv = 1, 3, 12), so this case runs the elastic loop on zero eigenstrain
     "v2_nonsq": (24, 2, (0.015, 0.015, -0.01), 0.005, {}, "rich"),
+    # radix-3 lengths with elasticity: 24^3 and 96^3 (the ensemble's middle size)
+    "v3_n24": (24, 3, (0.02, -0.01, -0.01), 0.005, {}, "rich"),
+    "v3_n96": (96, 3, (0.02, -0.01, -0.01), 0.005, {}, "rich"),
     "v1_e2": (16, 1, (0.01, 0.01, 0.01), 0.005, {"numel_in": 3}, "rich"),
     # 16 x 16 x 512: the length-512 z kernels (radix-4 plan of the fused elastic pass)
     "v3_z512": (16, 3, (0.02, -0.01, -0.01), 0.005, {"nz": 512}, "rich"),

@@ -54,7 +54,8 @@ def test_first_step_intermediates(gpu_lib, name):
 
 
 @pytest.mark.parametrize("name,nsteps", [("v3_tetra", 40), ("v3_inhomog", 20), ("v2_twophase", 30),
-                                         ("v3_noise", 60), ("v1_e2", 20), ("v12_rot", 10)])
+                                         ("v3_noise", 60), ("v1_e2", 20), ("v12_rot", 10),
+                                         ("v3_n24", 20)])
 def test_multi_step(gpu_lib, name, nsteps):
     p, phi, cons = make_case(name, seed=1)
     o = Oracle(p, 4)
@@ -131,7 +132,49 @@ def test_mixed_vs_fp64(gpu_lib):
         assert abs(r["caviter"][0] - c0) < 1e-10
 
 
-@pytest.mark.parametrize("name", ["v3_tetra", "v2_nonsq", "v3_inhomog"])
+@pytest.mark.parametrize("name", ["v3_inhomog", "v2_nonsq", "v3_n24", "v1_e2", "v12_rot", "v2_twophase"])
+def test_mixed_cases(gpu_lib, name):
+    """MIXED against FP64 on the other kernel combinations: inhomogeneous
+    moduli (unfused x path), a non-cubic grid, three solutes, twelve variants
+    (unfused x path), three phases."""
+    p, phi, cons = make_case(name, seed=6)
+    out = {}
+    for prec in (MM_FP64, MM_MIXED):
+        with Simulation(p, prec) as s:
+            s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+            rows = s.step(10)
+            out[prec] = (s.download(), rows)
+    a, b = out[MM_FP64][0], out[MM_MIXED][0]
+    for k in ("phi", "cons", "u"):
+        if not np.any(a[k]):  # no eigenstrain: zero displacement in both
+            assert not np.any(b[k]), k
+            continue
+        l2 = np.linalg.norm(b[k] - a[k]) / np.linalg.norm(a[k])
+        assert l2 < 1e-4, f"{k} rel L2 {l2:.2e}"
+    # the mean is held in FP64; what is left is the FP32 rounding of the
+    # stored deviation, at most 2^-24 of its largest value
+    c0 = cons.mean(axis=(1, 2, 3))
+    for r in out[MM_MIXED][1]:
+        for e in range(p.numel):
+            bound = 2.0 ** -24 * np.abs(cons[e] - c0[e]).max()
+            assert abs(r["caviter"][e] - c0[e]) < bound
+
+
+def test_mixed_mass_does_not_drift(gpu_lib):
+    """MIXED over 200 steps: the k = 0 mode of the stored deviation c - cbar
+    is reset by the Cahn-Hilliard update, so the FP32 rounding of the
+    transforms does not accumulate into the mean (left to accumulate it
+    reaches ~4e-9 here by step 200; reset it stays below ~4e-10)."""
+    p, phi, cons = make_case("v3_tetra", seed=6)
+    c0 = cons.mean()
+    with Simulation(p, MM_MIXED) as s:
+        s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+        rows = s.step(200)
+    drift = np.array([r["caviter"][0] - c0 for r in rows])
+    assert np.abs(drift).max() < 2.0 ** -26 * np.abs(cons - c0).max()
+
+
+@pytest.mark.parametrize("name", ["v3_tetra", "v3_n24", "v3_inhomog"])
 def test_strain_field(gpu_lib, name):
     """The strain field (kept as x half-spectra between steps on the fused x
     path) against the engineering strain of the oracle's displacement
