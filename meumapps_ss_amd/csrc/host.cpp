@@ -201,7 +201,8 @@ void noise_add(int32_t* idum, int32_t* state, double sd, int V, size_t n3, doubl
     return;
   }
   const int64_t m1 = 2147483563, m2 = 2147483399, a1 = 40014, a2 = 40692;
-  const int32_t imm1 = (int32_t)(m1 - 1), ndiv = 1 + imm1 / 32;
+  const int32_t imm1 = (int32_t)(m1 - 1);
+  constexpr int32_t kNdiv = 1 + (2147483563 - 1) / 32;
   const double am = (double)(1.0f / (float)m1), rnmx = 1.0 - 1.2e-7;
   size_t done = 0;  // draws produced so far, in (point, variant) order
   if (g.idum <= 0) {
@@ -227,29 +228,39 @@ void noise_add(int32_t* idum, int32_t* state, double sd, int V, size_t n3, doubl
     // 2. shuffle, uniforms, polar pairs, in order; stop after the last needed draw
     // (branch-free: the sign test of iy and the acceptance test are data
     // dependent and would mispredict about half and a fifth of the time)
+    // The table lives in a local array and the streams behind restrict
+    // pointers, so that its stores do not order the stream loads; iy >= 0
+    // here, so the table index is an unsigned division by the constant kNdiv.
     const size_t need = N - done;
     size_t used = 0, cnt = 0;
-    int32_t iy = g.iy;
+    int32_t iv[32];
+    for (int j = 0; j < 32; ++j) iv[j] = g.iv[j];
+    const int32_t* __restrict S1 = s1.data();
+    const int32_t* __restrict S2 = s2.data();
+    double* __restrict PV = pv2.data();
+    double* __restrict PR = prsq.data();
+    uint32_t iy = (uint32_t)g.iy;
     while (used + 2 <= U && cnt < need) {
       double u[2];
       for (int h = 0; h < 2; ++h) {
         const size_t n = used + h;
-        const int32_t j = iy / ndiv;
-        iy = g.iv[j] - s2[n];
-        g.iv[j] = s1[n];
-        iy += iy < 1 ? imm1 : 0;
-        const double x = am * (double)iy;
+        const uint32_t j = iy / (uint32_t)kNdiv;
+        int32_t t = iv[j] - S2[n];
+        iv[j] = S1[n];
+        t += t < 1 ? imm1 : 0;
+        iy = (uint32_t)t;
+        const double x = am * (double)t;
         u[h] = x < rnmx ? x : rnmx;
       }
       used += 2;
       const double v1 = 2.0 * u[0] - 1.0, v2 = 2.0 * u[1] - 1.0;
-      const double v1s = v1 * v1, v2s = v2 * v2;
-      const double rsq = v1s + v2s;
-      pv2[cnt] = v2;
-      prsq[cnt] = rsq;
+      const double rsq = v1 * v1 + v2 * v2;
+      PV[cnt] = v2;
+      PR[cnt] = rsq;
       cnt += (rsq > 0.0) & (rsq < 1.0);
     }
-    g.iy = iy;
+    for (int j = 0; j < 32; ++j) g.iv[j] = iv[j];
+    g.iy = (int32_t)iy;
     g.idum = s1[used - 1];
     g.idum2 = s2[used - 1];
     // 3. the Gaussian numbers, added in (point, variant) order
