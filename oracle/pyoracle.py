@@ -26,6 +26,34 @@ def build() -> str:
     return LIB
 
 
+# The reference's own ran_2.f (self-contained) compiled with flang where the
+# reference tree is present; draw-for-draw check of the ran_2 restatements.
+REF_SRC = "/root/reference/ran_2.f"
+REF_RAN2 = os.path.join(HERE, "_ref", "libran2_ref.so")
+
+
+def build_ref():
+    """Build oracle/_ref/libran2_ref.so from the reference's ran_2.f; None
+    where the reference tree is absent."""
+    if not os.path.exists(REF_SRC):
+        return None
+    r = subprocess.run(["make", "-s", "-C", HERE, "ref"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("reference ran_2 build failed:\n" + r.stderr)
+    return REF_RAN2
+
+
+def load_ref_ran2():
+    """ctypes handle of the reference ran_2 (real*8 function ran_2(idum),
+    SAVEd state), or None where it has not been built."""
+    if not os.path.exists(REF_RAN2):
+        return None
+    lib = ctypes.CDLL(REF_RAN2)
+    lib.ran_2_.restype = c_double
+    lib.ran_2_.argtypes = [POINTER(ctypes.c_int32)]
+    return lib
+
+
 def load():
     global _lib
     if _lib is not None:
