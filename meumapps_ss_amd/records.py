@@ -14,9 +14,12 @@ sequential file.  With one process (ist = (1,1,1), ien = (Nx,Ny,Nz)):
 These are exactly the arrays ``mm_upload`` takes and ``mm_download`` returns
 (component-fastest phi and Cons, then u1..u3 and the nine gradients in record
 order).  Records are framed by 4-byte little-endian length markers before and
-after the data, as gfortran, flang and nvfortran write them by default; a
-record longer than 2**31 - 9 bytes is split into subrecords whose markers
-carry a negative sign where the record continues (gfortran's convention).
+after the data, as gfortran, flang and nvfortran write them by default (the
+reference's data_ext_modular.f built with flang reads these files).  A record
+longer than 2**31 - 9 bytes is split into subrecords whose markers carry a
+negative sign where the record continues: gfortran's convention, which
+flang's runtime does not read; nvfortran's form for such records is not
+checked.
 
 ``step_NNNNNNN/data_extract_mult.in`` (:500-505, :1077-1083) holds
 ``Nx Ny Nz nprocs vartot numel`` and ``dims`` (list-directed).  fort.98 rows

@@ -27,14 +27,17 @@ def build() -> str:
 
 
 # The reference's own ran_2.f (self-contained) compiled with flang where the
-# reference tree is present; draw-for-draw check of the ran_2 restatements.
+# reference tree is present: draw-for-draw check of the ran_2 restatements.
 REF_SRC = "/root/reference/ran_2.f"
 REF_RAN2 = os.path.join(HERE, "_ref", "libran2_ref.so")
+# and its reader of the data_save records, data_ext_modular.f
+REF_DATA_EXT = os.path.join(HERE, "_ref", "data_ext_ref")
 
 
 def build_ref():
-    """Build oracle/_ref/libran2_ref.so from the reference's ran_2.f; None
-    where the reference tree is absent."""
+    """Build oracle/_ref (the reference's ran_2.f as a library and its
+    data_ext_modular.f as a program); None where the reference tree is
+    absent."""
     if not os.path.exists(REF_SRC):
         return None
     r = subprocess.run(["make", "-s", "-C", HERE, "ref"], capture_output=True, text=True)
