@@ -1,5 +1,5 @@
 /*
- * oracle.c -- TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ * oracle.c -- TEST INFRASTRUCTURE ONLY (the oracle of the parity tests, never the product).
  *
  * CPU binary64 restatement of one MEUMAPPS-SS semi-implicit spectral step.
  * The reference (Fortran + P3DFFT + FFTW + LINPACK + MPI) cannot be built in

@@ -1,4 +1,4 @@
-"""Pins of the CPU oracle (the parity checker) against things the oracle's own
+"""Pins of the CPU oracle (the parity reference of the tests) against things the oracle's own
 formulas do not restate: numpy's pocketfft and brute-force DFTs, closed-form
 single-mode solutions, thermodynamic and mechanical identities, and
 invariants.  Each check is chosen so that a plausible mistake (a dropped term,

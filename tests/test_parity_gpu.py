@@ -1,6 +1,6 @@
 """FP64 GPU step against the CPU binary64 oracle on identical seeded inputs.
 
-Bar (north star): every order-parameter and composition field within 1e-10
+Bar: every order-parameter and composition field within 1e-10
 max-abs after the stated number of steps; the displacement records, elastic
 energy and intermediate fields of the first step to the same bar relative to
 their scale.  MIXED is checked against the FP64 GPU path: relative L2
