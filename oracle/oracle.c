@@ -830,58 +830,63 @@ static void err_sums(oracle_t *o, double err[6]) {
   free(pf);
 }
 
+/* zeroth-order displacement at step == elastep0 from o->Pphi: FFT of P_phi,
+ * R with the real*8 truncation of sum_0 (ModElastic.f:29,151), u = Qinv R,
+ * u(k=0) = 0, u and its nine gradients to real space (:105-236) */
+static void eldis_zeroth(oracle_t *o) {
+  const int V = o->V, NPH = o->NPH, H = o->H, ny = o->ny, nt = o->nthreads;
+  long n3 = o->n3, nc = o->nc;
+  cplx *su = o->sw;           /* 3 spectra of u            */
+  cplx *sg = o->sw + 3 * nc;  /* 9 spectra of grad u       */
+  cplx *sp = (cplx *)malloc(sizeof(cplx) * V * nc);
+  for (int v = 0; v < V; ++v) f_trans(o, o->Pphi + v * n3, sp + v * nc);
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (long idx = 0; idx < nc; ++idx) {
+    int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+    double kf[3] = {o->kfx[kx], o->kfy[ky], o->kfz[kz]};
+    if (kx == 0 && ky == 0 && kz == 0) {
+      for (int c = 0; c < 3; ++c) su[c * nc + idx] = 0.0;
+      for (int c = 0; c < 9; ++c) sg[c * nc + idx] = 0.0;
+      continue;
+    }
+    double Ri[3] = {0, 0, 0}; /* R(i) = i * Ri[i] */
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        int ij = voigt(i, j);
+        double sum0 = 0.0;
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            int kl = voigt(k, l);
+            int ivn = 0;
+            for (int ph = 2; ph <= NPH; ++ph) {
+              ivn += o->p.nvar[ph - 2];
+              for (int iv = ivn; iv <= ivn + o->p.nvar[ph - 1] - 1; ++iv)
+                sum0 = sum0 + creal(sp[(iv - 1) * nc + idx]) * TP(o, kl, iv - 1) * CE(o, ij, kl, ph);
+            }
+          }
+        Ri[i] = Ri[i] + sum0 * kf[j];
+      }
+    const double *Q = o->qinv + 9 * idx;
+    for (int i = 0; i < 3; ++i) {
+      /* Q(i,1)*R(1) + Q(i,2)*R(2) + Q(i,3)*R(3), R purely imaginary */
+      double ui = Q[i] * Ri[0] + Q[i + 3] * Ri[1] + Q[i + 6] * Ri[2];
+      cplx uh = I * ui;
+      su[i * nc + idx] = uh;
+      for (int d = 0; d < 3; ++d) sg[(d * 3 + i) * nc + idx] = mul_i(uh, kf[d]);
+    }
+  }
+  free(sp);
+  for (int c = 0; c < 3; ++c) inv_trans(o, su + c * nc, o->u + c * n3);
+  for (int c = 0; c < 9; ++c) inv_trans(o, sg + c * nc, o->gu + c * n3);
+}
+
 /* eldis, ModElastic.f:11-489 (+ elener) */
 int oracle_eldis(oracle_t *o, int *n_iter, double *cfac_out) {
   const int V = o->V, NPH = o->NPH, H = o->H, ny = o->ny, nt = o->nthreads;
   long n3 = o->n3, nc = o->nc;
   int rc;
   if (!o->qinv_ready && (rc = build_qinv(o))) return rc;
-  cplx *su = o->sw;           /* 3 spectra of u            */
-  cplx *sg = o->sw + 3 * nc;  /* 9 spectra of grad u       */
-  if (o->step == o->p.elastep0) {
-    /* zeroth order: FFT of P_phi, R with the real*8 truncation of sum_0
-     * (ModElastic.f:29,151), u = Qinv R, u(k=0) = 0 (:105-236) */
-    cplx *sp = (cplx *)malloc(sizeof(cplx) * V * nc);
-    for (int v = 0; v < V; ++v) f_trans(o, o->Pphi + v * n3, sp + v * nc);
-#pragma omp parallel for num_threads(nt) schedule(static)
-    for (long idx = 0; idx < nc; ++idx) {
-      int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
-      double kf[3] = {o->kfx[kx], o->kfy[ky], o->kfz[kz]};
-      if (kx == 0 && ky == 0 && kz == 0) {
-        for (int c = 0; c < 3; ++c) su[c * nc + idx] = 0.0;
-        for (int c = 0; c < 9; ++c) sg[c * nc + idx] = 0.0;
-        continue;
-      }
-      double Ri[3] = {0, 0, 0}; /* R(i) = i * Ri[i] */
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-          int ij = voigt(i, j);
-          double sum0 = 0.0;
-          for (int k = 0; k < 3; ++k)
-            for (int l = 0; l < 3; ++l) {
-              int kl = voigt(k, l);
-              int ivn = 0;
-              for (int ph = 2; ph <= NPH; ++ph) {
-                ivn += o->p.nvar[ph - 2];
-                for (int iv = ivn; iv <= ivn + o->p.nvar[ph - 1] - 1; ++iv)
-                  sum0 = sum0 + creal(sp[(iv - 1) * nc + idx]) * TP(o, kl, iv - 1) * CE(o, ij, kl, ph);
-              }
-            }
-          Ri[i] = Ri[i] + sum0 * kf[j];
-        }
-      const double *Q = o->qinv + 9 * idx;
-      for (int i = 0; i < 3; ++i) {
-        /* Q(i,1)*R(1) + Q(i,2)*R(2) + Q(i,3)*R(3), R purely imaginary */
-        double ui = Q[i] * Ri[0] + Q[i + 3] * Ri[1] + Q[i + 6] * Ri[2];
-        cplx uh = I * ui;
-        su[i * nc + idx] = uh;
-        for (int d = 0; d < 3; ++d) sg[(d * 3 + i) * nc + idx] = mul_i(uh, kf[d]);
-      }
-    }
-    free(sp);
-    for (int c = 0; c < 3; ++c) inv_trans(o, su + c * nc, o->u + c * n3);
-    for (int c = 0; c < 9; ++c) inv_trans(o, sg + c * nc, o->gu + c * n3);
-  }
+  if (o->step == o->p.elastep0) eldis_zeroth(o);
   double newfac = 0.5, cfac = 1.0, err[6], err_old[6] = {0};
   int niter = 0, converged = 0, total = 0;
   const int vmap[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
@@ -982,6 +987,18 @@ int oracle_eldis(oracle_t *o, int *n_iter, double *cfac_out) {
   if (n_iter) *n_iter = total;
   if (cfac_out) *cfac_out = cfac;
   return elener(o);
+}
+
+/* the zeroth-order guess alone, from given P_phi fields (V*N3): u (3*N3)
+ * and its gradients (9*N3, d_d u_i at d*3+i); either output may be NULL */
+int oracle_elastic_zeroth(oracle_t *o, const double *pphi, double *u, double *gradu) {
+  int rc;
+  if (!o->qinv_ready && (rc = build_qinv(o))) return rc;
+  memcpy(o->Pphi, pphi, sizeof(double) * o->V * o->n3);
+  eldis_zeroth(o);
+  if (u) memcpy(u, o->u, sizeof(double) * 3 * o->n3);
+  if (gradu) memcpy(gradu, o->gu, sizeof(double) * 9 * o->n3);
+  return 0;
 }
 
 int oracle_step(oracle_t *o, mm_diag *d) {

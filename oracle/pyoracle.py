@@ -80,6 +80,7 @@ def load():
         "oracle_invert": (c_int, [D, c_int, c_int]),
         "oracle_kaxis": (None, [c_int, c_int, D, D]),
         "oracle_eldis": (c_int, [c_void_p, POINTER(c_int), D]),
+        "oracle_elastic_zeroth": (c_int, [c_void_p, D, D, D]),
         "oracle_ran2": (c_double, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32)]),
         "oracle_langevin": (c_double, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), c_double]),
         "oracle_set_noise": (c_int, [c_void_p, c_double, ctypes.c_int32, POINTER(ctypes.c_int32)]),
@@ -184,6 +185,16 @@ class Oracle:
         pn = np.empty((self.V, *self.shape))
         self.lib.oracle_get_noise(self.h, ctypes.byref(idum), st, _p(pn))
         return idum.value, list(st), pn
+
+    def elastic_zeroth(self, pphi):
+        """The zeroth-order guess alone from P_phi fields (V, nz, ny, nx):
+        returns (u (3, ...), gradu (9, ...))."""
+        pphi = np.ascontiguousarray(pphi, dtype=np.float64)
+        u = np.empty((3, *self.shape)); gu = np.empty((9, *self.shape))
+        rc = self.lib.oracle_elastic_zeroth(self.h, _p(pphi), _p(u), _p(gu))
+        if rc:
+            raise RuntimeError(f"oracle_elastic_zeroth error {rc}")
+        return u, gu
 
     def eldis(self):
         n = c_int(0)

@@ -231,12 +231,14 @@ def test_allen_cahn_linear_mode():
 
 
 # ------------------------------------------------------------ elasticity
-def _stress_and_divergence(p, st, phi):
-    """sigma = C (eps(u) + e_mean - sum P eps0) for homogeneous C; returns max
-    |k_j sigma_ij(k)| over non-Nyquist k relative to max |k||sigma(k)|."""
+def _stress_and_divergence(p, st, phi, P=None):
+    """sigma = C (eps(u) - sum P eps0) for homogeneous C (P = p(phi) unless
+    given); returns max |k_j sigma_ij(k)| over non-Nyquist k relative to
+    max |k||sigma(k)|."""
     C = p.cela_array()[1]
     V = p.vartot
-    P = phi ** 3 * (6 * phi ** 2 - 15 * phi + 10)
+    if P is None:
+        P = phi ** 3 * (6 * phi ** 2 - 15 * phi + 10)
     t = p.trans_ppt_array()
     g = st["gradu"]  # g[d*3 + i] = d_d u_i
     eps = np.empty((6,) + phi.shape[1:])
@@ -282,6 +284,31 @@ def test_elastic_solution_is_in_equilibrium(nvar, tstr, hi):
     assert rows[0]["n_elastic"] == 4  # converged at once, minimum of 4 iterations
     st = o.get_state()
     assert _stress_and_divergence(p, st, phi) < 1e-12
+
+
+@pytest.mark.parametrize("nvar,tstr", [(1, (0.02, -0.01, 0.005)), (3, (0.02, -0.01, -0.01))])
+def test_zeroth_order_guess_is_the_equilibrium_of_the_even_part(nvar, tstr):
+    """The zeroth-order guess of step == elastep0 (ModElastic.f:105-236) sums
+    only Re(P_hat) (sum_0 is real*8, :29,151) and takes R = i kf . sum_0.  Re
+    P_hat is the transform of the even part P_e(x) = (P(x) + P(-x))/2, so u0
+    is the equilibrium displacement of P_e: C(eps(u0) - eps0 P_e) is
+    divergence-free in every non-Nyquist mode, and against P itself it is
+    not (the odd part is dropped).  Pins the truncation, the sign of R and
+    Q(k) of the guess."""
+    from parity_cases import smooth_field
+
+    n = 16
+    p = setup(model_inputs(n, nvar, tstr, 0.005))
+    rng = np.random.default_rng(21)
+    phi = np.stack([smooth_field(rng, (n, n, n), 0.0, 0.9) for _ in range(nvar)])
+    P = phi ** 3 * (6 * phi ** 2 - 15 * phi + 10)
+    o = Oracle(p, 2)
+    o.set_state(phi=phi, cons=np.full((1, n, n, n), 0.4))
+    _, gu = o.elastic_zeroth(P)
+    r = (-np.arange(n)) % n
+    Pe = 0.5 * (P + P[:, r][:, :, r][:, :, :, r])
+    assert _stress_and_divergence(p, dict(gradu=gu), phi, Pe) < 1e-12
+    assert _stress_and_divergence(p, dict(gradu=gu), phi, P) > 1e-3
 
 
 def test_heterogeneous_fixed_point_is_in_equilibrium():
