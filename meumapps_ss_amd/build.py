@@ -52,7 +52,32 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
+    drv = build_fortran_driver()
+    if verbose and drv:
+        print(f"built {drv}", file=sys.stderr)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FDIR = os.path.join(PKG, "fortran")
+DRIVER = os.path.join(FDIR, "mm_step_driver")
+
+
+def build_fortran_driver():
+    """The Fortran step driver (fortran/mm_abi.f90 + fortran/mm_step_driver.f90)
+    linked against the library with an rpath relative to itself; None where
+    flang is absent."""
+    if not os.path.exists(FLANG):
+        return None
+    mods = os.path.join(BUILD, "fmod")
+    os.makedirs(mods, exist_ok=True)
+    cmd = [FLANG, "-O2", "-J", mods, os.path.join(FDIR, "mm_abi.f90"), os.path.join(FDIR, "mm_step_driver.f90"),
+           "-o", DRIVER + ".tmp", "-L", PKG, "-lmeumapps_ss", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=mods)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed for the step driver:\n{r.stderr[-6000:]}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
