@@ -535,7 +535,7 @@ __global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
 k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<T>* in, C2<T>* work, size_t nc,
           size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T scale, double cfac_arg,
           const double* __restrict__ cfac_dev, const int* __restrict__ gate) {
-  constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
   __shared__ C2<T> lds[TileLM<T, P>::LDS];
   // twiddle tables staged in LDS: the 12 transforms read them at every stage
   __shared__ C2<T> twl[L], twn[2 * L];
@@ -1974,7 +1974,6 @@ template <typename T> int finish_replay(mm_ctx* c, const double* st, mm_diag* d)
 }
 
 int graph_build(mm_ctx* c, hipStream_t cap);
-int graph_ready(mm_ctx* c);
 
 template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
   const bool regular = c->step > c->prm.elastep0 && c->step > c->prm.noise_step;
@@ -2024,7 +2023,7 @@ void free_all(mm_ctx* c) {
 }
 
 // upload helpers ---------------------------------------------------------
-template <typename T> int put_field(mm_ctx* c, void* dst, const std::vector<double>& h) {
+template <typename T> int put_field(mm_ctx*, void* dst, const std::vector<double>& h) {
   if (sizeof(T) == 8) {
     MM_HIP(hipMemcpy(dst, h.data(), h.size() * 8, hipMemcpyHostToDevice));
   } else {
@@ -2034,7 +2033,7 @@ template <typename T> int put_field(mm_ctx* c, void* dst, const std::vector<doub
   }
   return 0;
 }
-template <typename T> int get_field(mm_ctx* c, const void* src, std::vector<double>& h) {
+template <typename T> int get_field(mm_ctx*, const void* src, std::vector<double>& h) {
   if (sizeof(T) == 8) {
     MM_HIP(hipMemcpy(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost));
   } else {
