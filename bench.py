@@ -39,7 +39,11 @@ def step_bytes_per_cell(V, E, n, w):
     return w * (2 * nfft + npw)
 
 
-def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind, graph=False):
+# kernel classes timed one after another after the timed region (mm_probe kinds)
+PASS_KINDS = {"x_r2c": 0, "y": 1, "x_c2r": 3, "chem": 4, "elener": 6, "zf_other": 9, "x_polar": 10}
+
+
+def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind, graph=False, pass_steps=0):
     import numpy as np
     import torch
 
@@ -69,9 +73,22 @@ def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind, graph=False
     by = ctypes.c_double()
     _lib.check(lib.mm_probe_read(sim.ctx, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(by)), "probe")
     state = sim.download()
+    # after the timed region: live HIP-event timing of the other kernel classes,
+    # pass_steps steps each (the state keeps advancing; it is not compared)
+    kernels = {}
+    for name, kind in (PASS_KINDS.items() if pass_steps > 0 else []):
+        _lib.check(lib.mm_probe(sim.ctx, kind), "mm_probe")
+        sim.step(pass_steps, diag=False)
+        kms, knl, kby = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+        _lib.check(lib.mm_probe_read(sim.ctx, ctypes.byref(kms), ctypes.byref(knl), ctypes.byref(kby)), "probe")
+        if knl.value:
+            avg = kms.value / knl.value
+            gbs = kby.value / knl.value / (avg * 1e-3) / 1e9
+            kernels[name] = {"avg_launch_ms": avg, "launches": knl.value, "bytes_per_launch": kby.value / knl.value,
+                             "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
     sim.close()
     return dict(seconds=t1 - t0, rows=rows, probe_ms=ms.value, probe_launches=nl.value,
-                probe_bytes=by.value, state=state, tuning=tuning)
+                probe_bytes=by.value, state=state, tuning=tuning, kernels=kernels)
 
 
 def run_ensemble(steps, warmup, graph=True, per_size=16):
@@ -153,6 +170,9 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with three 1 GiB device copies (torch.neg, float64) that calibrate HBM "
                          "byte counters of a profiler run")
+    ap.add_argument("--pass-steps", type=int, default=3,
+                    help="after the timed region, steps per kernel class timed live for the per-kernel "
+                         "HBM rates (0 skips them)")
     ap.add_argument("--probe", default="zf_elastic",
                     choices=["x_r2c", "y", "z", "x_c2r", "chem", "polar", "elener", "ks_elastic",
                              "zf_elastic", "zf_other", "x_polar"])
@@ -177,7 +197,7 @@ def main():
              "zf_elastic": 8, "zf_other": 9, "x_polar": 10}
     kind = kinds[args.probe]
 
-    r64 = run_gpu(p, phi, cons, MM_FP64, args.steps, args.warmup, kind, args.graph)
+    r64 = run_gpu(p, phi, cons, MM_FP64, args.steps, args.warmup, kind, args.graph, args.pass_steps)
     cups64 = n3 * args.steps / r64["seconds"]
     n_el = [r["n_elastic"] for r in r64["rows"]]
     n_avg = sum(n_el) / len(n_el)
@@ -215,12 +235,14 @@ def main():
                            "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": args.probe,
                            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
                            "launches": r64["probe_launches"]}
+    if r64["kernels"]:
+        out["kernels"] = r64["kernels"]
     bpc = step_bytes_per_cell(3, 1, n_avg, 8)
     out["step_roofline"] = {"bytes_per_cell_update": bpc,
                             "achieved_GBs": bpc * cups64 / 1e9,
                             "frac": bpc * cups64 / 1e9 / HBM_PEAK_GBS}
     if not args.no_mixed:
-        r32 = run_gpu(p, phi, cons, MM_MIXED, args.steps, args.warmup, kind, args.graph)
+        r32 = run_gpu(p, phi, cons, MM_MIXED, args.steps, args.warmup, kind, args.graph, args.pass_steps)
         cups32 = n3 * args.steps / r32["seconds"]
         l2 = {}
         for k in ("phi", "cons"):
@@ -234,6 +256,8 @@ def main():
                         "probe_avg_launch_ms": (r32["probe_ms"] / r32["probe_launches"]
                                                 if r32["probe_launches"] else None),
                         "step_roofline_frac": bpc32 * cups32 / 1e9 / HBM_PEAK_GBS}
+        if r32["kernels"]:
+            out["mixed"]["kernels"] = r32["kernels"]
     if args.ensemble_steps > 0:
         eg = run_ensemble(args.ensemble_steps, 2, graph=True)
         ee = run_ensemble(args.ensemble_steps, 2, graph=False)
