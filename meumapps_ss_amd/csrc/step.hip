@@ -1327,6 +1327,15 @@ struct mm_ctx {
   // fused path: spectra between the x and y passes (S layout), kStageN of them;
   // the y pass moves them to and from the column-tiled layout of the z kernels
   void* sstage = nullptr;
+  // eager steps on the fused path run the Cahn-Hilliard update (which needs only
+  // c and the fluxes of the chemistry) on a second stream, beside the elastic
+  // solve, with its own work spectra (4E) and stage (4); MM_CH_STREAM=0 keeps
+  // it on the main stream
+  bool ch_async = false;
+  bool ch_launched = false;
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_head = nullptr, ev_ch = nullptr;
+  void* sch = nullptr;
   // with the fused x path: the strain between steps, as x half-spectra (S
   // layout after the inverse y pass, 6 components); c->strain is then filled
   // from it only on demand (field access, checkpoints)
@@ -1491,27 +1500,35 @@ template <typename T> int inv(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf
 
 // x and y passes only (the z pass is fused with a k-space operation): real
 // fields -> S spectra in the stage -> column-tiled spectra out[]
-template <typename T> int fwd_xy(mm_ctx* c, const T* const* in, C2<T>* const* out, int nf) {
+// (on stream s, default c->s, through a stage of nst spectra, default the
+// context's kStageN)
+template <typename T>
+int fwd_xy(mm_ctx* c, const T* const* in, C2<T>* const* out, int nf, hipStream_t s = nullptr,
+           C2<T>* stg = nullptr, int nst = kStageN) {
   const T scale = (T)(1.0 / (double)(float)c->n3);
-  C2<T>* stg = (C2<T>*)c->sstage;
+  if (!s) s = c->s;
+  if (!stg) stg = (C2<T>*)c->sstage;
   void* sp[kStageN];
-  for (int j = 0; j < kStageN; ++j) sp[j] = stg + (size_t)j * c->nc;
-  for (int f0 = 0; f0 < nf; f0 += kStageN) {
-    const int g = std::min(kStageN, nf - f0);
-    MM_TRY(c->fft.x_r2c<T>(in + f0, sp, g, c->s, scale));
-    MM_TRY(c->fft.y_pass_t<T>(sp, (void* const*)(out + f0), g, -1, 1, c->s));
+  for (int j = 0; j < nst; ++j) sp[j] = stg + (size_t)j * c->nc;
+  for (int f0 = 0; f0 < nf; f0 += nst) {
+    const int g = std::min(nst, nf - f0);
+    MM_TRY(c->fft.x_r2c<T>(in + f0, sp, g, s, scale));
+    MM_TRY(c->fft.y_pass_t<T>(sp, (void* const*)(out + f0), g, -1, 1, s));
   }
   return 0;
 }
 // column-tiled spectra sp[] -> S spectra in the stage -> real fields out[]
-template <typename T> int inv_yx(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf) {
-  C2<T>* stg = (C2<T>*)c->sstage;
+template <typename T>
+int inv_yx(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf, hipStream_t s = nullptr, C2<T>* stg = nullptr,
+           int nst = kStageN) {
+  if (!s) s = c->s;
+  if (!stg) stg = (C2<T>*)c->sstage;
   void* st[kStageN];
-  for (int j = 0; j < kStageN; ++j) st[j] = stg + (size_t)j * c->nc;
-  for (int f0 = 0; f0 < nf; f0 += kStageN) {
-    const int g = std::min(kStageN, nf - f0);
-    MM_TRY(c->fft.y_pass_t<T>((void* const*)(sp + f0), st, g, +1, 2, c->s));
-    MM_TRY(c->fft.x_c2r<T>(st, out + f0, g, c->s));
+  for (int j = 0; j < nst; ++j) st[j] = stg + (size_t)j * c->nc;
+  for (int f0 = 0; f0 < nf; f0 += nst) {
+    const int g = std::min(nst, nf - f0);
+    MM_TRY(c->fft.y_pass_t<T>((void* const*)(sp + f0), st, g, +1, 2, s));
+    MM_TRY(c->fft.x_c2r<T>(st, out + f0, g, s));
   }
   return 0;
 }
@@ -1670,6 +1687,7 @@ template <typename T> int elastic_loop_eager(mm_ctx* c, Ctl& h, bool spectral) {
 // head: phi_hat, gradient block, chemistry (Multi_MEUMAPPS.f:575-845)
 template <typename T> int step_head(mm_ctx* c) {
   const size_t n3 = c->n3, nc = c->nc;
+  c->ch_launched = false;  // set by the eager step once the update is on the second stream
   const int V = c->V, E = c->E;
   const Kern<T>& K = kern<T>(c);
   T* phi = (T*)c->phi;
@@ -1735,6 +1753,43 @@ int finalize_clamp(mm_ctx* c, int nblk) {
   return 0;
 }
 
+// Cahn-Hilliard update (Multi_MEUMAPPS.f:910-949) on stream s: work[4e] = c_hat,
+// work[4e+1..3] = F_hat (4E spectra), the fused passes staging through stg
+// (nst spectra; null = the context's stage); writes c
+template <typename T> int ch_update(mm_ctx* c, hipStream_t s, C2<T>* work, C2<T>* stg, int nst) {
+  const size_t n3 = c->n3, nc = c->nc;
+  const int E = c->E;
+  const T scale = (T)(1.0 / (double)(float)n3);
+  for (int e = 0; e < E; ++e) {
+    const T* ip[4] = {(T*)c->cons + (size_t)e * n3, (T*)c->flux + (size_t)(3 * e) * n3,
+                      (T*)c->flux + (size_t)(3 * e + 1) * n3, (T*)c->flux + (size_t)(3 * e + 2) * n3};
+    void* op[4] = {work + (size_t)(4 * e) * nc, work + (size_t)(4 * e + 1) * nc, work + (size_t)(4 * e + 2) * nc,
+                   work + (size_t)(4 * e + 3) * nc};
+    if (c->zfused) {
+      MM_TRY(fwd_xy<T>(c, ip, (C2<T>* const*)op, 4, s, stg, nst));
+    } else {
+      MM_TRY(c->fft.forward<T>(ip, op, 4, s, scale));
+    }
+  }
+  if (c->zfused) {
+    OpCH<T> opch{c->dp, work, nc};
+    opch.gate = c->gate;
+    MM_TRY(launch_zfused<T>(c->fft, opch, E, s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 5 * E, nullptr));
+  } else {
+    hipLaunchKernelGGL(k_ks_ch<T>, dim3(c->nblk_k, E), dim3(kBlock), 0, s, c->dp, work, nc);
+  }
+  MM_HIP(hipGetLastError());
+  C2<T>* sp[MM_MAX_EL];
+  T* rp[MM_MAX_EL];
+  for (int e = 0; e < E; ++e) { sp[e] = work + (size_t)(4 * e) * nc; rp[e] = (T*)c->cons + (size_t)e * n3; }
+  if (c->zfused) {
+    MM_TRY(inv_yx<T>(c, sp, rp, E, s, stg, nst));
+  } else {
+    MM_TRY(inv<T>(c, sp, rp, E));
+  }
+  return 0;
+}
+
 // tail: elener (when the elastic solve ran), Cahn-Hilliard, Allen-Cahn, clamp
 // rec_fd / rec_phipre (FP64 tape): copies of f_diff after elener and of phi
 // before the clamp
@@ -1767,36 +1822,12 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
     hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(kBlock), 0, c->s, c->dres + R_EL, (size_t)1, 0.0, c->gate);
     MM_HIP(hipGetLastError());
   }
-  // Cahn-Hilliard (:910-949): work[4e] = c_hat, work[4e+1..3] = F_hat
-  {
-    const T scale = (T)(1.0 / (double)(float)n3);
-    for (int e = 0; e < E; ++e) {
-      const T* ip[4] = {(T*)c->cons + (size_t)e * n3, (T*)c->flux + (size_t)(3 * e) * n3,
-                        (T*)c->flux + (size_t)(3 * e + 1) * n3, (T*)c->flux + (size_t)(3 * e + 2) * n3};
-      void* op[4] = {work + (size_t)(4 * e) * nc, work + (size_t)(4 * e + 1) * nc, work + (size_t)(4 * e + 2) * nc,
-                     work + (size_t)(4 * e + 3) * nc};
-      if (c->zfused) {
-        MM_TRY(fwd_xy<T>(c, ip, (C2<T>* const*)op, 4));
-      } else {
-        MM_TRY(c->fft.forward<T>(ip, op, 4, c->s, scale));
-      }
-    }
-    if (c->zfused) {
-      OpCH<T> opch{c->dp, work, nc};
-      opch.gate = c->gate;
-      MM_TRY(launch_zfused<T>(c->fft, opch, E, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 5 * E, nullptr));
-    } else {
-      hipLaunchKernelGGL(k_ks_ch<T>, dim3(c->nblk_k, E), dim3(kBlock), 0, c->s, c->dp, work, nc);
-    }
-    MM_HIP(hipGetLastError());
-    C2<T>* sp[MM_MAX_EL];
-    T* rp[MM_MAX_EL];
-    for (int e = 0; e < E; ++e) { sp[e] = work + (size_t)(4 * e) * nc; rp[e] = (T*)c->cons + (size_t)e * n3; }
-    if (c->zfused) {
-      MM_TRY(inv_yx<T>(c, sp, rp, E));
-    } else {
-      MM_TRY(inv<T>(c, sp, rp, E));
-    }
+  // Cahn-Hilliard (:910-949), unless it already ran beside the elastic solve
+  if (c->ch_launched) {
+    MM_HIP(hipStreamWaitEvent(c->s, c->ev_ch, 0));
+    c->ch_launched = false;
+  } else {
+    MM_TRY(ch_update<T>(c, c->s, work, nullptr, kStageN));
   }
   // Allen-Cahn (:955-1001)
   {
@@ -1914,6 +1945,18 @@ template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
   }
   set_gate(c, nullptr);
   MM_TRY(step_head<T>(c));
+  // the Cahn-Hilliard update needs only c and the fluxes: on the second stream
+  // it runs beside the elastic solve (joined before the Allen-Cahn update).
+  // Not while a probe times one of its kernel classes on the main stream.
+  const int pk = c->probe.kind;
+  if (c->ch_async && pk != MM_PROBE_X_R2C && pk != MM_PROBE_Y && pk != MM_PROBE_X_C2R && pk != MM_PROBE_ZF_OTHER) {
+    C2<T>* w = (C2<T>*)c->sch;
+    MM_HIP(hipEventRecord(c->ev_head, c->s));
+    MM_HIP(hipStreamWaitEvent(c->s2, c->ev_head, 0));
+    MM_TRY(ch_update<T>(c, c->s2, w, w + (size_t)4 * c->E * c->nc, 4));
+    MM_HIP(hipEventRecord(c->ev_ch, c->s2));
+    c->ch_launched = true;
+  }
   // the host draws overlap the head on the device
   if (noisy) MM_TRY(noise_advance<T>(c));
   const bool elastic = c->step >= c->prm.elastep0;
@@ -2017,6 +2060,10 @@ void free_all(mm_ctx* c) {
   if (c->noise_pin) (void)hipHostFree(c->noise_pin);
   if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->ev) (void)hipEventDestroy(c->ev);
+  if (c->ev_head) (void)hipEventDestroy(c->ev_head);
+  if (c->ev_ch) (void)hipEventDestroy(c->ev_ch);
+  if (c->sch) (void)hipFree(c->sch);
+  if (c->s2) (void)hipStreamDestroy(c->s2);
   c->probe.destroy();
   if (c->s) (void)hipStreamDestroy(c->s);
   c->fft.destroy();
@@ -2262,6 +2309,16 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   if (!rc && hipHostMalloc((void**)&c->hres, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = MM_ERR_NOMEM;
   if (!rc && hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) rc = MM_ERR_HIP;
   if (!rc && hipEventCreateWithFlags(&c->ev, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
+  {
+    const char* e = std::getenv("MM_CH_STREAM");
+    c->ch_async = c->zfused && c->E <= MM_MAX_EL && !(e && e[0] == '0');
+  }
+  if (!rc && c->ch_async) {
+    rc = dalloc(&c->sch, (size_t)(4 * E + 4) * nc * cb);
+    if (!rc && hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) rc = MM_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&c->ev_head, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&c->ev_ch, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
+  }
   if (!rc && hipMemcpy(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess) rc = MM_ERR_HIP;
   if (!rc && c->autotune) rc = c->fft.autotune(c->s);
   if (!rc) {
