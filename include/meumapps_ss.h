@@ -17,7 +17,10 @@
  *   - Every function returns 0 on success or one of the MM_ERR_* codes; the
  *     positive codes reuse the reference's err_flag values (ModFunc.f:44-72,
  *     Manual p.9) so a driver can print the reference's STOP messages.
- *   - One context = one device, one HIP stream; a context is not reentrant.
+ *   - One context = one device; a context is not reentrant.  It owns its HIP
+ *     streams: the step runs on one; an eager step on the fused path runs the
+ *     Cahn-Hilliard update on a second one, joined within the step
+ *     (MM_CH_STREAM=0 in the environment at mm_create keeps it on the first).
  */
 #ifndef MEUMAPPS_SS_H
 #define MEUMAPPS_SS_H
