@@ -1327,18 +1327,15 @@ struct mm_ctx {
   // fused path: spectra between the x and y passes (S layout), kStageN of them;
   // the y pass moves them to and from the column-tiled layout of the z kernels
   void* sstage = nullptr;
-  // eager steps on the fused path run the head (phi_hat, gradient block,
-  // chemistry) and the Cahn-Hilliard update on a second stream beside the
-  // elastic solve, which needs neither; their own work spectra (nsch_work),
-  // stage (kStageN) and partial sums; MM_CH_STREAM=0 keeps them on the main
-  // stream
+  // eager steps on the fused path run the Cahn-Hilliard update (which needs only
+  // c and the fluxes of the chemistry) on a second stream, beside the elastic
+  // solve, with its own work spectra (4E) and stage (4); MM_CH_STREAM=0 keeps
+  // it on the main stream
   bool ch_async = false;
   bool ch_launched = false;
   hipStream_t s2 = nullptr;
   hipEvent_t ev_head = nullptr, ev_ch = nullptr;
   void* sch = nullptr;
-  int nsch_work = 0;
-  double* part2 = nullptr;
   // with the fused x path: the strain between steps, as x half-spectra (S
   // layout after the inverse y pass, 6 components); c->strain is then filled
   // from it only on demand (field access, checkpoints)
@@ -1462,14 +1459,12 @@ int dalloc(void** p, size_t bytes) {
 template <typename T> T* F(void* base, size_t n, int k) { return (T*)base + (size_t)k * n; }
 template <typename T> C2<T>* S(void* base, size_t nc, int k) { return (C2<T>*)base + (size_t)k * nc; }
 
-int finalize(mm_ctx* c, int nblk, int K, std::initializer_list<int> ops, int slot0, hipStream_t s = nullptr,
-             double* part = nullptr) {
+int finalize(mm_ctx* c, int nblk, int K, std::initializer_list<int> ops, int slot0) {
   FinalizeOps fo;
   int i = 0;
   for (int o : ops) fo.op[i++] = o;
   for (; i < 8; ++i) fo.op[i] = 0;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s ? s : c->s, part ? part : c->part, nblk, K, fo,
-                     c->dres + slot0, c->gate);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, c->s, c->part, nblk, K, fo, c->dres + slot0, c->gate);
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -1690,31 +1685,22 @@ template <typename T> int elastic_loop_eager(mm_ctx* c, Ctl& h, bool spectral) {
 
 // -------------------------------------------------------------- one step
 // head: phi_hat, gradient block, chemistry (Multi_MEUMAPPS.f:575-845)
-// head of the step on stream s (default c->s) with the work spectra, stage
-// and partial-sum buffer given (default the context's); the error flag is
-// cleared here only on the main stream (the caller clears it before forking)
-template <typename T>
-int step_head(mm_ctx* c, hipStream_t s = nullptr, C2<T>* work = nullptr, C2<T>* stg = nullptr,
-              double* part = nullptr) {
+template <typename T> int step_head(mm_ctx* c) {
   const size_t n3 = c->n3, nc = c->nc;
+  c->ch_launched = false;  // set by the eager step once the update is on the second stream
   const int V = c->V, E = c->E;
   const Kern<T>& K = kern<T>(c);
   T* phi = (T*)c->phi;
   C2<T>* sphi = (C2<T>*)c->sphi;
-  if (!s) {
-    s = c->s;
-    c->ch_launched = false;  // set by the eager step when the head runs on the second stream
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(kBlock), 0, c->s, c->derr, (size_t)1, 0, c->gate);
-    MM_HIP(hipGetLastError());
-  }
-  if (!work) work = (C2<T>*)c->swork;
-  if (!part) part = c->part;
+  C2<T>* work = (C2<T>*)c->swork;
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(kBlock), 0, c->s, c->derr, (size_t)1, 0, c->gate);
+  MM_HIP(hipGetLastError());
   // 1. phi_hat (:577-578); kept for the Allen-Cahn update
   if (c->zfused) {
     const T* pin[MM_MAX_VAR];
     C2<T>* pout[MM_MAX_VAR];
     for (int v = 0; v < V; ++v) { pin[v] = phi + (size_t)v * n3; pout[v] = sphi + (size_t)v * nc; }
-    MM_TRY(fwd_xy<T>(c, pin, pout, V, s, stg));
+    MM_TRY(fwd_xy<T>(c, pin, pout, V));
   } else {
     MM_TRY(fwd<T>(c, phi, n3, sphi, V));
   }
@@ -1725,9 +1711,9 @@ int step_head(mm_ctx* c, hipStream_t s = nullptr, C2<T>* work = nullptr, C2<T>* 
     if (c->zfused) {
       OpGrad<T> op{c->dp, sphi, work, nc, v0};
       op.gate = c->gate;
-      MM_TRY(launch_zfused<T>(c->fft, op, g, s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 6 * g, nullptr));
+      MM_TRY(launch_zfused<T>(c->fft, op, g, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 6 * g, nullptr));
     } else {
-      hipLaunchKernelGGL(k_ks_grad<T>, dim3(c->nblk_k, g), dim3(kBlock), 0, s, c->dp, sphi, work, v0, nc);
+      hipLaunchKernelGGL(k_ks_grad<T>, dim3(c->nblk_k, g), dim3(kBlock), 0, c->s, c->dp, sphi, work, v0, nc);
     }
     MM_HIP(hipGetLastError());
     C2<T>* sp[16];
@@ -1739,18 +1725,18 @@ int step_head(mm_ctx* c, hipStream_t s = nullptr, C2<T>* work = nullptr, C2<T>* 
       rp[4 * j + 3] = (T*)c->kdel + (size_t)v * n3;
     }
     if (c->zfused) {
-      MM_TRY(inv_yx<T>(c, sp, rp, 4 * g, s, stg));
+      MM_TRY(inv_yx<T>(c, sp, rp, 4 * g));
     } else {
       MM_TRY(inv<T>(c, sp, rp, 4 * g));
     }
   }
   // 3. chemistry (:634-845)
-  c->probe.begin(MM_PROBE_CHEM, s);
-  K.chem(dim3(c->nblk_pw), s, c->dp, phi, (T*)c->gphi, (T*)c->kdel, (T*)c->cons, (T*)c->fdiff, (T*)c->flux,
-         part, c->derr, n3);
-  c->probe.end(MM_PROBE_CHEM, s, (double)n3 * sizeof(T) * (6 * V + 4 * E));
+  c->probe.begin(MM_PROBE_CHEM, c->s);
+  K.chem(dim3(c->nblk_pw), c->s, c->dp, phi, (T*)c->gphi, (T*)c->kdel, (T*)c->cons, (T*)c->fdiff, (T*)c->flux,
+         c->part, c->derr, n3);
+  c->probe.end(MM_PROBE_CHEM, c->s, (double)n3 * sizeof(T) * (6 * V + 4 * E));
   MM_HIP(hipGetLastError());
-  MM_TRY(finalize(c, c->nblk_pw, 3, {0, 0, 0}, R_FINT, s, part));
+  MM_TRY(finalize(c, c->nblk_pw, 3, {0, 0, 0}, R_FINT));
   return 0;
 }
 
@@ -1815,12 +1801,6 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   C2<T>* sphi = (C2<T>*)c->sphi;
   C2<T>* work = (C2<T>*)c->swork;
   const dim3 gp(c->nblk_pw);
-  // the head and the Cahn-Hilliard update may have run on the second stream
-  const bool ch_done = c->ch_launched;
-  if (ch_done) {
-    MM_HIP(hipStreamWaitEvent(c->s, c->ev_ch, 0));
-    c->ch_launched = false;
-  }
   if (elastic) {
     c->probe.begin(MM_PROBE_ELENER, c->s);
     int nblk = c->nblk_pw;
@@ -1843,7 +1823,12 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
     MM_HIP(hipGetLastError());
   }
   // Cahn-Hilliard (:910-949), unless it already ran beside the elastic solve
-  if (!ch_done) MM_TRY(ch_update<T>(c, c->s, work, nullptr, kStageN));
+  if (c->ch_launched) {
+    MM_HIP(hipStreamWaitEvent(c->s, c->ev_ch, 0));
+    c->ch_launched = false;
+  } else {
+    MM_TRY(ch_update<T>(c, c->s, work, nullptr, kStageN));
+  }
   // Allen-Cahn (:955-1001)
   {
     C2<T>* sp[MM_MAX_VAR];
@@ -1959,25 +1944,18 @@ template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
     c->noise_live = false;
   }
   set_gate(c, nullptr);
-  // the elastic solve needs only phi and the strain, so the head (phi_hat,
-  // gradient block, chemistry) and the Cahn-Hilliard update run on the second
-  // stream beside it; joined before elener.  Not while a probe times one of
-  // their kernel classes.
+  MM_TRY(step_head<T>(c));
+  // the Cahn-Hilliard update needs only c and the fluxes: on the second stream
+  // it runs beside the elastic solve (joined before the Allen-Cahn update).
+  // Not while a probe times one of its kernel classes on the main stream.
   const int pk = c->probe.kind;
-  if (c->ch_async && pk != MM_PROBE_X_R2C && pk != MM_PROBE_Y && pk != MM_PROBE_X_C2R && pk != MM_PROBE_ZF_OTHER &&
-      pk != MM_PROBE_CHEM) {
+  if (c->ch_async && pk != MM_PROBE_X_R2C && pk != MM_PROBE_Y && pk != MM_PROBE_X_C2R && pk != MM_PROBE_ZF_OTHER) {
     C2<T>* w = (C2<T>*)c->sch;
-    C2<T>* stg = w + (size_t)c->nsch_work * c->nc;
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(kBlock), 0, c->s, c->derr, (size_t)1, 0, c->gate);
-    MM_HIP(hipGetLastError());
     MM_HIP(hipEventRecord(c->ev_head, c->s));
     MM_HIP(hipStreamWaitEvent(c->s2, c->ev_head, 0));
-    MM_TRY(step_head<T>(c, c->s2, w, stg, c->part2));
-    MM_TRY(ch_update<T>(c, c->s2, w, stg, kStageN));
+    MM_TRY(ch_update<T>(c, c->s2, w, w + (size_t)4 * c->E * c->nc, 4));
     MM_HIP(hipEventRecord(c->ev_ch, c->s2));
     c->ch_launched = true;
-  } else {
-    MM_TRY(step_head<T>(c));
   }
   // the host draws overlap the head on the device
   if (noisy) MM_TRY(noise_advance<T>(c));
@@ -2085,7 +2063,6 @@ void free_all(mm_ctx* c) {
   if (c->ev_head) (void)hipEventDestroy(c->ev_head);
   if (c->ev_ch) (void)hipEventDestroy(c->ev_ch);
   if (c->sch) (void)hipFree(c->sch);
-  if (c->part2) (void)hipFree(c->part2);
   if (c->s2) (void)hipStreamDestroy(c->s2);
   c->probe.destroy();
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -2337,12 +2314,7 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
     c->ch_async = c->zfused && c->E <= MM_MAX_EL && !(e && e[0] == '0');
   }
   if (!rc && c->ch_async) {
-    // the gradient block's groups of up to 4 variants (4 spectra each) or the
-    // Cahn-Hilliard update's 4E spectra, then a stage of kStageN
-    const int G = std::max(1, std::min({V, c->nwork / 4, 4}));
-    c->nsch_work = std::max(4 * G, 4 * E);
-    rc = dalloc(&c->sch, (size_t)(c->nsch_work + kStageN) * nc * cb);
-    rc = rc ? rc : dalloc((void**)&c->part2, (size_t)kMaxPartSlots * kMaxPartBlocks * sizeof(double));
+    rc = dalloc(&c->sch, (size_t)(4 * E + 4) * nc * cb);
     if (!rc && hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) rc = MM_ERR_HIP;
     if (!rc && hipEventCreateWithFlags(&c->ev_head, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
     if (!rc && hipEventCreateWithFlags(&c->ev_ch, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
