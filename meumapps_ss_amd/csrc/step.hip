@@ -1506,6 +1506,7 @@ template <typename T>
 int fwd_xy(mm_ctx* c, const T* const* in, C2<T>* const* out, int nf, hipStream_t s = nullptr,
            C2<T>* stg = nullptr, int nst = kStageN) {
   const T scale = (T)(1.0 / (double)(float)c->n3);
+  if (nst < 1 || nst > kStageN) return MM_ERR_ARG;
   if (!s) s = c->s;
   if (!stg) stg = (C2<T>*)c->sstage;
   void* sp[kStageN];
@@ -1521,6 +1522,7 @@ int fwd_xy(mm_ctx* c, const T* const* in, C2<T>* const* out, int nf, hipStream_t
 template <typename T>
 int inv_yx(mm_ctx* c, C2<T>* const* sp, T* const* out, int nf, hipStream_t s = nullptr, C2<T>* stg = nullptr,
            int nst = kStageN) {
+  if (nst < 1 || nst > kStageN) return MM_ERR_ARG;
   if (!s) s = c->s;
   if (!stg) stg = (C2<T>*)c->sstage;
   void* st[kStageN];
