@@ -253,3 +253,21 @@ def test_backtracking_matches_oracle(gpu_lib):
     ro = o.get_state()
     for k in ("phi", "cons"):
         assert np.abs(st[k] - ro[k]).max() < TOL, k
+
+
+@pytest.mark.parametrize("name,prec", [("v3_tetra", MM_FP64), ("v3_n24", MM_MIXED), ("v1_e2", MM_FP64)])
+def test_second_stream_gives_identical_bits(gpu_lib, name, prec, monkeypatch):
+    """The Cahn-Hilliard update on its own stream (the default for eager steps)
+    and on the main stream (MM_CH_STREAM=0) give the same bits."""
+    p, phi, cons = make_case(name, seed=8)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MM_CH_STREAM", flag)
+        with Simulation(p, prec) as s:
+            s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+            rows = s.step(8)
+            out[flag] = (s.download(), [r["n_elastic"] for r in rows], [r["caviter"][0] for r in rows])
+    a, b = out["0"], out["1"]
+    for k in ("phi", "cons", "u", "gradu", "el_en"):
+        assert np.array_equal(a[0][k], b[0][k]), k
+    assert a[1] == b[1] and a[2] == b[2]
