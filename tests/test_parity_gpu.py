@@ -271,3 +271,25 @@ def test_second_stream_gives_identical_bits(gpu_lib, name, prec, monkeypatch):
     for k in ("phi", "cons", "u", "gradu", "el_en"):
         assert np.array_equal(a[0][k], b[0][k]), k
     assert a[1] == b[1] and a[2] == b[2]
+
+
+@pytest.mark.parametrize("prec", [MM_FP64, MM_MIXED])
+def test_fixed_plans_are_bitwise_repeatable(gpu_lib, prec, monkeypatch):
+    """Two contexts with the default FFT plans (MM_AUTOTUNE=0) give the same
+    bits over 20 steps of the 64^3 one-variant workload (fixed-order
+    reductions, no atomics; the Cahn-Hilliard update on its own stream)."""
+    from meumapps_ss_amd.configs import initial_fields, workload
+
+    monkeypatch.setenv("MM_AUTOTUNE", "0")
+    w = workload(1, n=64)
+    p = w.params()
+    phi, cons = initial_fields(w)
+    outs = []
+    for _ in range(2):
+        with Simulation(p, prec) as s:
+            s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+            rows = s.step(21)
+            outs.append((s.download(), [(r["fint"], r["fch"], r["elen"], r["n_elastic"]) for r in rows]))
+    for k in ("phi", "cons", "u", "gradu", "el_en"):
+        assert np.array_equal(outs[0][0][k], outs[1][0][k]), k
+    assert outs[0][1] == outs[1][1]
