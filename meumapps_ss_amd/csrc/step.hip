@@ -1949,9 +1949,10 @@ template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
   MM_TRY(step_head<T>(c));
   // the Cahn-Hilliard update needs only c and the fluxes: on the second stream
   // it runs beside the elastic solve (joined before the Allen-Cahn update).
-  // Not while a probe times one of its kernel classes on the main stream.
+  // While a probe times a kernel class other than the elastic z kernel the
+  // update stays on the main stream, so that the probed kernels run alone.
   const int pk = c->probe.kind;
-  if (c->ch_async && pk != MM_PROBE_X_R2C && pk != MM_PROBE_Y && pk != MM_PROBE_X_C2R && pk != MM_PROBE_ZF_OTHER) {
+  if (c->ch_async && (pk < 0 || pk == MM_PROBE_ZF_ELASTIC)) {
     C2<T>* w = (C2<T>*)c->sch;
     MM_HIP(hipEventRecord(c->ev_head, c->s));
     MM_HIP(hipStreamWaitEvent(c->s2, c->ev_head, 0));
