@@ -19,8 +19,9 @@
  *     Manual p.9) so a driver can print the reference's STOP messages.
  *   - One context = one device; a context is not reentrant.  It owns its HIP
  *     streams: the step runs on one; an eager step on the fused path runs the
- *     Cahn-Hilliard update on a second one, joined within the step
- *     (MM_CH_STREAM=0 in the environment at mm_create keeps it on the first).
+ *     Cahn-Hilliard update on a second one, joined within the step (by default
+ *     up to 256^3 points; MM_CH_STREAM=0 / 1 in the environment at mm_create
+ *     forces it off / on).
  */
 #ifndef MEUMAPPS_SS_H
 #define MEUMAPPS_SS_H
