@@ -68,58 +68,165 @@ static int plan_init(plan1d *p, int n) {
 
 static void plan_free(plan1d *p) { free(p->w); p->w = NULL; }
 
-static inline cplx tw(const plan1d *p, long e, int sign) {
-  cplx w = p->w[e % p->n];
-  return sign < 0 ? w : conj(w);
-}
-
-/* DFT of in[0], in[is], ..., in[(n-1) is] into out[0..n); sign -1 forward. */
-static void fft_rec(cplx *out, const cplx *in, int n, int is, const plan1d *p, int fi,
-                    int wstride, int sign) {
-  if (n == 1) { out[0] = in[0]; return; }
-  int r = p->f[fi], m = n / r;
-  for (int q = 0; q < r; ++q) fft_rec(out + q * m, in + q * is, m, is * r, p, fi + 1, wstride * r, sign);
-  cplx t[64];
-  for (int k = 0; k < m; ++k) {
-    t[0] = out[k];
-    for (int q = 1; q < r; ++q) t[q] = out[q * m + k] * tw(p, (long)q * k * wstride, sign);
-    if (r == 2) {
-      out[k] = t[0] + t[1];
-      out[k + m] = t[0] - t[1];
-    } else if (r == 4) {
-      cplx a0 = t[0] + t[2], a1 = t[0] - t[2], a2 = t[1] + t[3], a3 = t[1] - t[3];
-      cplx ja3 = (sign < 0) ? -I * a3 : I * a3;
-      out[k] = a0 + a2;
-      out[k + m] = a1 + ja3;
-      out[k + 2 * m] = a0 - a2;
-      out[k + 3 * m] = a1 - ja3;
-    } else {
-      for (int s = 0; s < r; ++s) {
-        cplx acc = t[0];
-        for (int q = 1; q < r; ++q) acc += t[q] * tw(p, (long)((q * s) % r) * m * wstride, sign);
-        out[k + s * m] = acc;
+#define FB 16
+/* Stockham autosort transform of FB lines at once, stored split and
+ * interleaved: re[e*FB + j], im[e*FB + j] is element e of line j.  Stage with
+ * radix r on the current length L (stride s = product of the radices done,
+ * m = L/r): for p < m, q < s the inputs x[q + s(p + m t)], t < r, go through an
+ * r-point DFT and the outputs, times w_L^(p t), land at y[q + s(r p + t)].
+ * sign -1 forward (FFTW's e^{-i...}), +1 backward.  Work arrays t_re/t_im
+ * (n*FB each) hold the ping-pong copy; the result ends in re/im. */
+static void fft_batch(double *re, double *im, double *t_re, double *t_im, const plan1d *p,
+                      int sign) {
+  const int n = p->n;
+  double *xr = re, *xi = im, *yr = t_re, *yi = t_im;
+  int L = n, s = 1;
+  const double sg = (double)sign;
+  const double s3 = 0.86602540378443864676; /* sin(2 pi / 3) */
+  for (int st = 0; st < p->nf; ++st) {
+    const int r = p->f[st], m = L / r, wst = n / L;
+    for (int pp = 0; pp < m; ++pp) {
+      double wr[64], wi[64];
+      for (int t = 1; t < r; ++t) {
+        cplx w = p->w[(long)pp * t * wst];
+        wr[t] = creal(w);
+        wi[t] = sg < 0 ? cimag(w) : -cimag(w);
+      }
+      for (int q = 0; q < s; ++q) {
+        const long ib = ((long)q + (long)s * pp) * FB, istep = (long)s * m * FB;
+        const long ob = ((long)q + (long)s * r * pp) * FB, ostep = (long)s * FB;
+        if (r == 4) {
+          for (int j = 0; j < FB; ++j) {
+            double a0r = xr[ib + j], a0i = xi[ib + j];
+            double a1r = xr[ib + istep + j], a1i = xi[ib + istep + j];
+            double a2r = xr[ib + 2 * istep + j], a2i = xi[ib + 2 * istep + j];
+            double a3r = xr[ib + 3 * istep + j], a3i = xi[ib + 3 * istep + j];
+            double b0r = a0r + a2r, b0i = a0i + a2i, b1r = a0r - a2r, b1i = a0i - a2i;
+            double b2r = a1r + a3r, b2i = a1i + a3i, b3r = a1r - a3r, b3i = a1i - a3i;
+            /* (sign i) * b3 */
+            double c3r = -sg * b3i, c3i = sg * b3r;
+            double u0r = b0r + b2r, u0i = b0i + b2i;
+            double u1r = b1r + c3r, u1i = b1i + c3i;
+            double u2r = b0r - b2r, u2i = b0i - b2i;
+            double u3r = b1r - c3r, u3i = b1i - c3i;
+            yr[ob + j] = u0r; yi[ob + j] = u0i;
+            if (pp == 0) {
+              yr[ob + ostep + j] = u1r; yi[ob + ostep + j] = u1i;
+              yr[ob + 2 * ostep + j] = u2r; yi[ob + 2 * ostep + j] = u2i;
+              yr[ob + 3 * ostep + j] = u3r; yi[ob + 3 * ostep + j] = u3i;
+            } else {
+              yr[ob + ostep + j] = u1r * wr[1] - u1i * wi[1];
+              yi[ob + ostep + j] = u1r * wi[1] + u1i * wr[1];
+              yr[ob + 2 * ostep + j] = u2r * wr[2] - u2i * wi[2];
+              yi[ob + 2 * ostep + j] = u2r * wi[2] + u2i * wr[2];
+              yr[ob + 3 * ostep + j] = u3r * wr[3] - u3i * wi[3];
+              yi[ob + 3 * ostep + j] = u3r * wi[3] + u3i * wr[3];
+            }
+          }
+        } else if (r == 2) {
+          for (int j = 0; j < FB; ++j) {
+            double a0r = xr[ib + j], a0i = xi[ib + j];
+            double a1r = xr[ib + istep + j], a1i = xi[ib + istep + j];
+            double u1r = a0r - a1r, u1i = a0i - a1i;
+            yr[ob + j] = a0r + a1r; yi[ob + j] = a0i + a1i;
+            if (pp == 0) {
+              yr[ob + ostep + j] = u1r; yi[ob + ostep + j] = u1i;
+            } else {
+              yr[ob + ostep + j] = u1r * wr[1] - u1i * wi[1];
+              yi[ob + ostep + j] = u1r * wi[1] + u1i * wr[1];
+            }
+          }
+        } else if (r == 3) {
+          for (int j = 0; j < FB; ++j) {
+            double a0r = xr[ib + j], a0i = xi[ib + j];
+            double a1r = xr[ib + istep + j], a1i = xi[ib + istep + j];
+            double a2r = xr[ib + 2 * istep + j], a2i = xi[ib + 2 * istep + j];
+            double tr = a1r + a2r, ti = a1i + a2i, dr = a1r - a2r, di = a1i - a2i;
+            double mr = a0r - 0.5 * tr, mi = a0i - 0.5 * ti;
+            /* u1 = m + sign*i*s3*d, u2 = m - sign*i*s3*d */
+            double er = -sg * s3 * di, ei = sg * s3 * dr;
+            double u1r = mr + er, u1i = mi + ei, u2r = mr - er, u2i = mi - ei;
+            yr[ob + j] = a0r + tr; yi[ob + j] = a0i + ti;
+            if (pp == 0) {
+              yr[ob + ostep + j] = u1r; yi[ob + ostep + j] = u1i;
+              yr[ob + 2 * ostep + j] = u2r; yi[ob + 2 * ostep + j] = u2i;
+            } else {
+              yr[ob + ostep + j] = u1r * wr[1] - u1i * wi[1];
+              yi[ob + ostep + j] = u1r * wi[1] + u1i * wr[1];
+              yr[ob + 2 * ostep + j] = u2r * wr[2] - u2i * wi[2];
+              yi[ob + 2 * ostep + j] = u2r * wi[2] + u2i * wr[2];
+            }
+          }
+        } else { /* any other radix: direct r-point DFT, then the twiddle */
+          const int wr_st = n / r; /* w_r^(ts) = w_n^(ts n/r) */
+          for (int j = 0; j < FB; ++j) {
+            for (int t = 0; t < r; ++t) {
+              double accr = 0.0, acci = 0.0;
+              for (int u = 0; u < r; ++u) {
+                cplx w = p->w[(long)((t * u) % r) * wr_st];
+                double cr = creal(w), ci = sg < 0 ? cimag(w) : -cimag(w);
+                double vr = xr[ib + u * istep + j], vi = xi[ib + u * istep + j];
+                accr += vr * cr - vi * ci;
+                acci += vr * ci + vi * cr;
+              }
+              if (pp != 0 && t != 0) {
+                double zr = accr * wr[t] - acci * wi[t];
+                acci = accr * wi[t] + acci * wr[t];
+                accr = zr;
+              }
+              yr[ob + t * ostep + j] = accr; yi[ob + t * ostep + j] = acci;
+            }
+          }
+        }
       }
     }
+    double *sw;
+    sw = xr; xr = yr; yr = sw;
+    sw = xi; xi = yi; yi = sw;
+    L = m;
+    s *= r;
+  }
+  if (xr != re) {
+    memcpy(re, xr, sizeof(double) * (size_t)n * FB);
+    memcpy(im, xi, sizeof(double) * (size_t)n * FB);
   }
 }
 
-/* transform `count` lines: line l starts at base + l_off(l), element stride es */
+/* transform `count` lines: line l starts at base + l_off(l), element stride es.
+ * Lines l, l+1, ... with consecutive offsets are gathered FB at a time so the
+ * strided reads touch whole cache lines. */
 static void fft_lines(cplx *data, const plan1d *p, int count, long (*off)(int, const int *),
                       const int *geo, long es, int sign, int nthreads) {
   int n = p->n;
+  int nblk = (count + FB - 1) / FB;
 #pragma omp parallel num_threads(nthreads)
   {
-    cplx *a = (cplx *)malloc(sizeof(cplx) * (size_t)n);
-    cplx *b = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    double *w = (double *)malloc(sizeof(double) * (size_t)n * FB * 4);
+    double *ar = w, *ai = w + (long)n * FB, *tr = w + 2L * n * FB, *ti = w + 3L * n * FB;
 #pragma omp for schedule(static)
-    for (int l = 0; l < count; ++l) {
-      cplx *base = data + off(l, geo);
-      for (int e = 0; e < n; ++e) a[e] = base[(long)e * es];
-      fft_rec(b, a, n, 1, p, 0, 1, sign);
-      for (int e = 0; e < n; ++e) base[(long)e * es] = b[e];
+    for (int blk = 0; blk < nblk; ++blk) {
+      int l0 = blk * FB, nb = count - l0 < FB ? count - l0 : FB;
+      long o0 = off(l0, geo);
+      int run = 1; /* lines l0..l0+nb-1 at consecutive offsets? */
+      for (int j = 1; j < nb; ++j) run &= (off(l0 + j, geo) == o0 + j);
+      for (int e = 0; e < n; ++e)
+        for (int j = nb; j < FB; ++j) ar[(long)e * FB + j] = ai[(long)e * FB + j] = 0.0;
+      for (int j = 0; j < nb; ++j) {
+        const cplx *src = data + (run ? o0 + j : off(l0 + j, geo));
+        for (int e = 0; e < n; ++e) {
+          cplx v = src[(long)e * es];
+          ar[(long)e * FB + j] = creal(v);
+          ai[(long)e * FB + j] = cimag(v);
+        }
+      }
+      fft_batch(ar, ai, tr, ti, p, sign);
+      for (int j = 0; j < nb; ++j) {
+        cplx *dst = data + (run ? o0 + j : off(l0 + j, geo));
+        for (int e = 0; e < n; ++e)
+          dst[(long)e * es] = CMPLX(ar[(long)e * FB + j], ai[(long)e * FB + j]);
+      }
     }
-    free(a);
-    free(b);
+    free(w);
   }
 }
 
@@ -442,19 +549,26 @@ int oracle_get_work(oracle_t *o, int which, double *out) {
 /* f_trans (ModFunc.f:15-26): r2c over x, c2c over y and z, then * ooNxyz */
 static void f_trans(oracle_t *o, const double *in, cplx *out) {
   int nx = o->nx, ny = o->ny, nz = o->nz, H = o->H, nt = o->nthreads;
-  int rows = ny * nz;
+  int rows = ny * nz, nblk = (rows + FB - 1) / FB;
 #pragma omp parallel num_threads(nt)
   {
-    cplx *a = (cplx *)malloc(sizeof(cplx) * nx), *b = (cplx *)malloc(sizeof(cplx) * nx);
+    double *w = (double *)malloc(sizeof(double) * (size_t)nx * FB * 4);
+    double *ar = w, *ai = w + (long)nx * FB, *tr = w + 2L * nx * FB, *ti = w + 3L * nx * FB;
 #pragma omp for schedule(static)
-    for (int r = 0; r < rows; ++r) {
-      const double *row = in + (long)r * nx;
-      for (int x = 0; x < nx; ++x) a[x] = row[x];
-      fft_rec(b, a, nx, 1, &o->px, 0, 1, -1);
-      cplx *orow = out + (long)r * H;
-      for (int k = 0; k < H; ++k) orow[k] = b[k];
+    for (int blk = 0; blk < nblk; ++blk) {
+      int r0 = blk * FB, nb = rows - r0 < FB ? rows - r0 : FB;
+      for (int x = 0; x < nx; ++x)
+        for (int j = 0; j < FB; ++j) {
+          ar[(long)x * FB + j] = j < nb ? in[(long)(r0 + j) * nx + x] : 0.0;
+          ai[(long)x * FB + j] = 0.0;
+        }
+      fft_batch(ar, ai, tr, ti, &o->px, -1);
+      for (int j = 0; j < nb; ++j) {
+        cplx *orow = out + (long)(r0 + j) * H;
+        for (int k = 0; k < H; ++k) orow[k] = CMPLX(ar[(long)k * FB + j], ai[(long)k * FB + j]);
+      }
     }
-    free(a); free(b);
+    free(w);
   }
   int geo[3] = {H, ny, nz};
   /* along y: line (z, kx), elements stride H */
@@ -475,22 +589,35 @@ static void inv_trans(oracle_t *o, cplx *in, double *out) {
   int geo[3] = {H, ny, nz};
   fft_lines(in, &o->pz, ny * H, off_z, geo, (long)H * ny, +1, nt);
   fft_lines(in, &o->py, nz * H, off_y, geo, H, +1, nt);
-  int rows = ny * nz;
+  int rows = ny * nz, nblk = (rows + FB - 1) / FB;
 #pragma omp parallel num_threads(nt)
   {
-    cplx *a = (cplx *)malloc(sizeof(cplx) * nx), *b = (cplx *)malloc(sizeof(cplx) * nx);
+    double *w = (double *)malloc(sizeof(double) * (size_t)nx * FB * 4);
+    double *ar = w, *ai = w + (long)nx * FB, *tr = w + 2L * nx * FB, *ti = w + 3L * nx * FB;
 #pragma omp for schedule(static)
-    for (int r = 0; r < rows; ++r) {
-      const cplx *irow = in + (long)r * H;
-      a[0] = creal(irow[0]);
-      for (int k = 1; k < H; ++k) a[k] = irow[k];
-      if ((nx & 1) == 0) a[nx / 2] = creal(irow[nx / 2]);
-      for (int k = H; k < nx; ++k) a[k] = conj(a[nx - k]);
-      fft_rec(b, a, nx, 1, &o->px, 0, 1, +1);
-      double *row = out + (long)r * nx;
-      for (int x = 0; x < nx; ++x) row[x] = creal(b[x]);
+    for (int blk = 0; blk < nblk; ++blk) {
+      int r0 = blk * FB, nb = rows - r0 < FB ? rows - r0 : FB;
+      for (int j = 0; j < FB; ++j) {
+        const cplx *irow = in + (long)(j < nb ? r0 + j : r0) * H;
+        /* Hermitian extension; the imaginary parts of kx = 0 and Nx/2 are dropped */
+        for (int k = 0; k < H; ++k) {
+          ar[(long)k * FB + j] = creal(irow[k]);
+          ai[(long)k * FB + j] = cimag(irow[k]);
+        }
+        ai[j] = 0.0;
+        if ((nx & 1) == 0) ai[(long)(nx / 2) * FB + j] = 0.0;
+        for (int k = H; k < nx; ++k) {
+          ar[(long)k * FB + j] = ar[(long)(nx - k) * FB + j];
+          ai[(long)k * FB + j] = -ai[(long)(nx - k) * FB + j];
+        }
+      }
+      fft_batch(ar, ai, tr, ti, &o->px, +1);
+      for (int j = 0; j < nb; ++j) {
+        double *row = out + (long)(r0 + j) * nx;
+        for (int x = 0; x < nx; ++x) row[x] = ar[(long)x * FB + j];
+      }
     }
-    free(a); free(b);
+    free(w);
   }
 }
 
@@ -986,6 +1113,24 @@ int oracle_eldis(oracle_t *o, int *n_iter, double *cfac_out) {
   }
   if (n_iter) *n_iter = total;
   if (cfac_out) *cfac_out = cfac;
+  return elener(o);
+}
+
+/* P_phi = p(phi) of the current phi (the interpolation of
+ * Multi_MEUMAPPS.f:653), without the rest of the chemistry */
+int oracle_set_pphi(oracle_t *o) {
+  long n3 = o->n3;
+  for (long i = 0; i < (long)o->V * n3; ++i) {
+    double fi = o->phi[i], fi_sq = fi * fi;
+    o->Pphi[i] = fi_sq * fi * (6.0 * fi_sq - 15.0 * fi + 10.0);
+  }
+  return 0;
+}
+
+/* elener alone (ModElastic.f:493-691) at the current phi, P_phi and grad u:
+ * el_en and el_grad of the state as it stands */
+int oracle_elener(oracle_t *o) {
+  memset(o->elgrad, 0, sizeof(double) * o->V * o->n3);
   return elener(o);
 }
 

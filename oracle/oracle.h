@@ -68,6 +68,11 @@ int oracle_get_noise(oracle_t *o, int32_t *idum, int32_t *state, double *phi_noi
 /* elastic solve alone (eldis + elener) on the current state, given P_phi
  * fields computed from phi; returns 0 or error and the iteration count */
 int oracle_eldis(oracle_t *o, int *n_iter, double *cfac);
+/* P_phi = p(phi) of the current phi alone */
+int oracle_set_pphi(oracle_t *o);
+/* elener alone at the current phi, P_phi and grad u (el_en: oracle_get_state,
+ * el_grad: oracle_get_work(4)) */
+int oracle_elener(oracle_t *o);
 /* the zeroth-order guess of step == elastep0 alone (ModElastic.f:105-236)
  * from given P_phi fields (V*N3): u (3*N3) and grad u (9*N3, d_d u_i at
  * d*3+i); either output may be NULL */

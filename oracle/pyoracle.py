@@ -15,7 +15,9 @@ import numpy as np
 from meumapps_ss_amd.params import MMDiag, MMParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# MM_ORACLE_LIB points the tests at another build of oracle.c (a copy with a
+# planted mistake, to show that the pins catch it)
+LIB = os.environ.get("MM_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 _lib = None
 
 
@@ -81,6 +83,8 @@ def load():
         "oracle_kaxis": (None, [c_int, c_int, D, D]),
         "oracle_eldis": (c_int, [c_void_p, POINTER(c_int), D]),
         "oracle_elastic_zeroth": (c_int, [c_void_p, D, D, D]),
+        "oracle_set_pphi": (c_int, [c_void_p]),
+        "oracle_elener": (c_int, [c_void_p]),
         "oracle_ran2": (c_double, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32)]),
         "oracle_langevin": (c_double, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), c_double]),
         "oracle_set_noise": (c_int, [c_void_p, c_double, ctypes.c_int32, POINTER(ctypes.c_int32)]),
@@ -195,6 +199,16 @@ class Oracle:
         if rc:
             raise RuntimeError(f"oracle_elastic_zeroth error {rc}")
         return u, gu
+
+    def set_pphi(self):
+        """P_phi = p(phi) of the current phi (input of eldis / elener)."""
+        self.lib.oracle_set_pphi(self.h)
+
+    def elener(self):
+        """elener alone at the current phi, P_phi and grad u; returns
+        (rc, el_en, el_grad)."""
+        rc = self.lib.oracle_elener(self.h)
+        return rc, self.get_state()["el_en"], self.get_work(4)
 
     def eldis(self):
         n = c_int(0)

@@ -377,6 +377,123 @@ def test_traceless_eigenstrain_switches_el_grad_off():
     assert np.abs(o.get_work(4)).max() == 0.0
 
 
+# eigenstrains with every Voigt component non-zero (tensor shear), trace > 0
+# so the transsum > 1e-9 guard of ModElastic.f:653 keeps el_grad on
+_SHEAR_EPS0 = ((0.012, -0.004, 0.006, 0.003, -0.002, 0.005),
+               (0.004, 0.010, -0.003, -0.004, 0.006, 0.002))
+
+
+def _elener_params(kind, n):
+    """dilat: one variant, dilatational, homogeneous moduli; shear: two
+    variants of one phase with full (shear-carrying) eigenstrains; twophase:
+    two precipitate phases, the first with its own cubic moduli, both with
+    shear-carrying eigenstrains."""
+    if kind == "dilat":
+        return setup(model_inputs(n, 1, (0.01, 0.01, 0.01), 0.005))
+    if kind == "shear":
+        p = setup(model_inputs(n, 2, (0.01, 0.01, 0.01), 0.005))
+    else:
+        p = setup(model_inputs(n, 1, (0.01, 0.01, 0.01), 0.005, numph=3,
+                               cela2={(1, 1): 320.0, (1, 2): 130.0, (4, 4): 70.0}))
+    for v in range(2):
+        for m in range(6):
+            p.trans_ppt[v * 6 + m] = _SHEAR_EPS0[v][m]
+    return p
+
+
+@pytest.mark.parametrize("kind", ["dilat", "shear", "twophase"])
+def test_el_grad_is_the_phi_derivative_of_el_en(kind):
+    """elener's el_grad (ModElastic.f:653-678) is the derivative of the local
+    elastic energy density el_en = 1/2 sigma : S(phi) : sigma with respect to
+    phi_v at fixed grad u: p'(phi_v) [(el_mat - el_ppt_v) - sigma : eps0_v]
+    (engineering shear, so the shear terms of sigma : eps0 count twice).
+    Checked against the central difference of the oracle's own el_en, point
+    by point, for a dilatational eigenstrain, shear-carrying eigenstrains and
+    a second precipitate phase with other moduli (el_mat != el_ppt).  A
+    flipped sign of the sigma : eps0 term, a dropped factor 2 on its shear
+    terms or a flipped (el_mat - el_ppt) fails by orders of magnitude."""
+    n = 8
+    p = _elener_params(kind, n)
+    V = p.vartot
+    rng = np.random.default_rng(31)
+    phi = rng.uniform(0.05, 0.45, (V, n, n, n))
+    gu = rng.uniform(-0.02, 0.02, (9, n, n, n))
+    cons = np.full((p.numel, n, n, n), 0.4)
+    o = Oracle(p, 1)
+
+    def run(ph):
+        o.set_state(phi=ph, cons=cons, gradu=gu)
+        o.set_pphi()
+        rc, en, eg = o.elener()
+        assert rc == 0
+        return en, eg
+
+    en0, eg0 = run(phi)
+    # el_en is the Hooke energy 1/2 eps_el : C(phi) : eps_el with the compliance mix
+    P = phi ** 3 * (6 * phi ** 2 - 15 * phi + 10)
+    S = p.sela_array()
+    Smix = (1 - P.sum(0))[..., None, None] * S[0]
+    ph_of_v = [ph for ph in range(1, p.numph) for _ in range(p.nvar[ph])]
+    for v in range(V):
+        Smix = Smix + P[v][..., None, None] * S[ph_of_v[v]]
+    t = p.trans_ppt_array()
+    w = np.array([1, 1, 1, 2, 2, 2.0])
+    g = gu
+    eps = np.stack([g[0], g[4], g[8], g[5] + g[7], g[2] + g[6], g[1] + g[3]], axis=-1)
+    eps = eps - np.einsum("v...,vm->...m", P, t) * w
+    sig = np.linalg.solve(Smix, eps[..., None])[..., 0]
+    assert np.abs(en0 - 0.5 * (sig * eps).sum(-1)).max() < 1e-12 * np.abs(en0).max()
+    h = 1e-5
+    for v in range(V):
+        dp = np.zeros_like(phi)
+        dp[v] = h
+        fd = (run(phi + dp)[0] - run(phi - dp)[0]) / (2 * h)
+        scale = np.abs(eg0[v]).max()
+        assert scale > 1e-3
+        assert np.abs(fd - eg0[v]).max() < 1e-7 * scale, (kind, v)
+
+
+@pytest.mark.parametrize("kind", ["dilat", "shear"])
+def test_el_grad_gives_the_equilibrium_energy_change(kind):
+    """Envelope theorem: at elastic equilibrium (homogeneous moduli, where
+    eldis converges to the exact equilibrium) the change of the total elastic
+    energy sum el_en under a local change delta phi equals
+    sum el_grad . delta phi, because the energy is stationary in u.  The
+    equilibrium is recomputed by eldis (ModElastic.f:241-481) at phi +- h
+    delta phi; el_grad comes from the base state only."""
+    from parity_cases import smooth_field
+
+    n = 16
+    p = _elener_params(kind, n)
+    V = p.vartot
+    rng = np.random.default_rng(33)
+    phi = np.stack([smooth_field(rng, (n, n, n), 0.0, 0.45) for _ in range(V)])
+    cons = np.full((p.numel, n, n, n), 0.4)
+    z = np.arange(n)
+    zz, yy, xx = np.meshgrid(z, z, z, indexing="ij")
+    o = Oracle(p, 2)
+
+    def solve(ph):
+        o.set_state(phi=ph, cons=cons, u=np.zeros((3, n, n, n)), gradu=np.zeros((9, n, n, n)))
+        o.set_step(p.elastep0 + 1)
+        o.set_pphi()
+        rc, nit, _ = o.eldis()
+        assert rc == 0
+        return o.get_state()["el_en"], o.get_work(4)
+
+    en0, eg0 = solve(phi)
+    for v in range(V):
+        c = (3 + 5 * v, 7, 10 - 2 * v)
+        r2 = (zz - c[0]) ** 2 + (yy - c[1]) ** 2 + (xx - c[2]) ** 2
+        bump = np.zeros_like(phi)
+        bump[v] = np.exp(-r2 / 8.0)
+        lin = float((eg0 * bump).sum())
+        h = 1e-5
+        fd = (solve(phi + h * bump)[0].sum() - solve(phi - h * bump)[0].sum()) / (2 * h)
+        assert abs(lin) > 1e-3
+        assert abs(fd - lin) < 1e-6 * abs(lin), (kind, v, fd, lin)
+
+
 # ------------------------------------------------------------ invariants
 def test_mass_clamp_and_lagged_diagnostics():
     from parity_cases import make_case
