@@ -39,8 +39,35 @@ def step_bytes_per_cell(V, E, n, w):
     return w * (2 * nfft + npw)
 
 
-# kernel classes timed one after another after the timed region (mm_probe kinds)
-PASS_KINDS = {"x_r2c": 0, "y": 1, "x_c2r": 3, "chem": 4, "elener": 6, "zf_other": 9, "x_polar": 10}
+# kernel classes (mm_probe kinds); the one named by --probe is timed inside the
+# timed region, the others one after another after it
+KINDS = {"x_r2c": 0, "y": 1, "z": 2, "x_c2r": 3, "chem": 4, "polar": 5, "elener": 6, "ks_elastic": 7,
+         "zf_elastic": 8, "zf_other": 9, "x_polar": 10}
+PASS_KINDS = ("x_r2c", "y", "x_c2r", "chem", "elener", "zf_elastic", "zf_other", "x_polar")
+# kernel symbol of each headline class (for the rocprofv3 counter runs)
+KERNEL_NAME = {"x_polar": "k_x_polar", "zf_elastic": "k_z_fused<OpElastic>"}
+
+
+def host_cpu():
+    """CPU model, logical CPUs this process may run on and the physical cores
+    among them (distinct (package, core) pairs)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            cores.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(cores)}
 
 
 def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind, graph=False, pass_steps=0):
@@ -76,15 +103,17 @@ def run_gpu(params, phi, cons, precision, steps, warmup, probe_kind, graph=False
     # after the timed region: live HIP-event timing of the other kernel classes,
     # pass_steps steps each (the state keeps advancing; it is not compared)
     kernels = {}
-    for name, kind in (PASS_KINDS.items() if pass_steps > 0 else []):
-        _lib.check(lib.mm_probe(sim.ctx, kind), "mm_probe")
+    for name in (PASS_KINDS if pass_steps > 0 else ()):
+        _lib.check(lib.mm_probe(sim.ctx, KINDS[name]), "mm_probe")
         sim.step(pass_steps, diag=False)
         kms, knl, kby = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
         _lib.check(lib.mm_probe_read(sim.ctx, ctypes.byref(kms), ctypes.byref(knl), ctypes.byref(kby)), "probe")
         if knl.value:
             avg = kms.value / knl.value
             gbs = kby.value / knl.value / (avg * 1e-3) / 1e9
-            kernels[name] = {"avg_launch_ms": avg, "launches": knl.value, "bytes_per_launch": kby.value / knl.value,
+            kernels[name] = {"avg_launch_ms": avg, "launches_per_step": knl.value / pass_steps,
+                             "ms_per_step": kms.value / pass_steps,
+                             "bytes_per_launch": kby.value / knl.value,
                              "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
     sim.close()
     return dict(seconds=t1 - t0, rows=rows, probe_ms=ms.value, probe_launches=nl.value,
@@ -174,9 +203,10 @@ def main():
     ap.add_argument("--pass-steps", type=int, default=3,
                     help="after the timed region, steps per kernel class timed live for the per-kernel "
                          "HBM rates (0 skips them)")
-    ap.add_argument("--probe", default="zf_elastic",
-                    choices=["x_r2c", "y", "z", "x_c2r", "chem", "polar", "elener", "ks_elastic",
-                             "zf_elastic", "zf_other", "x_polar"])
+    ap.add_argument("--probe", default="x_polar", choices=sorted(KINDS),
+                    help="kernel class timed live inside the timed region for the roofline line "
+                         "(default: the polarization pass k_x_polar, the kernel with the largest "
+                         "time per step)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build keeps the whole grid on a single MI355X "
@@ -194,9 +224,7 @@ def main():
     p = w.params()
     phi, cons = initial_fields(w)
     n3 = args.size ** 3
-    kinds = {"x_r2c": 0, "y": 1, "z": 2, "x_c2r": 3, "chem": 4, "polar": 5, "elener": 6, "ks_elastic": 7,
-             "zf_elastic": 8, "zf_other": 9, "x_polar": 10}
-    kind = kinds[args.probe]
+    kind = KINDS[args.probe]
 
     r64 = run_gpu(p, phi, cons, MM_FP64, args.steps, args.warmup, kind, args.graph, args.pass_steps)
     cups64 = n3 * args.steps / r64["seconds"]
@@ -226,16 +254,20 @@ def main():
         ach = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         traffic = None
         tj = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-        if args.probe == "zf_elastic" and args.size == 256 and os.path.exists(tj):
+        key = f"{KERNEL_NAME.get(args.probe, args.probe)} {args.size}^3 FP64"
+        if os.path.exists(tj):
             # HBM bytes per launch of this kernel on this workload from the committed
             # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
-            t = json.load(open(tj)).get("k_z_fused<double,256,OpElastic>")
+            t = json.load(open(tj)).get(key)
             if t:
                 traffic = t["fetch_bytes"] + t["write_bytes"]
         out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": args.probe,
+                           "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                           "kernel": KERNEL_NAME.get(args.probe, args.probe),
                            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
-                           "launches": r64["probe_launches"]}
+                           "launches_per_step": r64["probe_launches"] / args.steps,
+                           "ms_per_step": r64["probe_ms"] / args.steps,
+                           "share_of_step": r64["probe_ms"] / (1e3 * r64["seconds"])}
     if r64["kernels"]:
         out["kernels"] = r64["kernels"]
     bpc = step_bytes_per_cell(3, 1, n_avg, 8)
@@ -279,13 +311,16 @@ def main():
         torch.cuda.synchronize()
         out["calib_copy_bytes"] = {"read": a.numel() * 8, "write": b.numel() * 8}
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        hc = host_cpu()
+        # one OpenMP thread per physical core this process may use
+        threads = args.cpu_threads or hc["physical_cores"]
         secs = cpu_baseline(p, phi, cons, threads)
         out["cpu_baseline"] = {"value": n3 / secs, "unit": "cell-updates/s", "cores": threads,
                                "kind": "port",
                                "sample": f"2 steps of the {args.size}^3 3-variant workload (after the "
                                          f"elastep0 step) by the CPU binary64 oracle, {threads} OpenMP threads",
-                               "seconds_per_step": secs}
+                               "seconds_per_step": secs, "cpu_model": hc["model"],
+                               "physical_cores": hc["physical_cores"], "logical_cpus": hc["logical_cpus"]}
     print(json.dumps(out))
 
 

@@ -18,10 +18,9 @@
  *     positive codes reuse the reference's err_flag values (ModFunc.f:44-72,
  *     Manual p.9) so a driver can print the reference's STOP messages.
  *   - One context = one device; a context is not reentrant.  It owns its HIP
- *     streams: the step runs on one; an eager step on the fused path runs the
- *     Cahn-Hilliard update on a second one, joined within the step (by default
- *     up to 256^3 points; MM_CH_STREAM=0 / 1 in the environment at mm_create
- *     forces it off / on).
+ *     stream and runs the step on it; with MM_CH_STREAM=1 in the environment
+ *     at mm_create an eager step on the fused path runs the Cahn-Hilliard
+ *     update on a second stream of its own, joined within the step.
  */
 #ifndef MEUMAPPS_SS_H
 #define MEUMAPPS_SS_H
@@ -248,7 +247,9 @@ int mm_probe_read(mm_ctx *ctx, double *ms_total, int64_t *launches, double *byte
  * the fused path (forward + inverse with the change to the column-tiled
  * layout) (default, lean, 4-point) is timed on the grid's own size and the
  * fastest kept; set the environment variable MM_AUTOTUNE=0 to keep the
- * defaults.  variant[5] gets the chosen plan per pass, ms[15] the measured
+ * defaults.  The choice is made once per grid size, precision and device in a
+ * process and reused by every later context of that shape (so contexts of one
+ * process, ensemble members and single runs alike, round the same way).  variant[5] gets the chosen plan per pass, ms[15] the measured
  * kernel times (ms, 4 fields). */
 int mm_get_tuning(const mm_ctx *ctx, int32_t *variant, float *ms);
 

@@ -810,6 +810,7 @@ extern "C" int mm_state_load(mm_ctx* c, const void* src) {
 extern "C" int mm_adj_tape_step(mm_ctx* c, mm_diag* d) {
   if (!c || !c->adj) return MM_ERR_ARG;
   if (c->step <= c->prm.elastep0 || c->step <= c->prm.noise_step) return MM_ERR_ARG;
+  MM_TRY(qcheck_once(c));
   mm_adj* a = c->adj;
   const size_t n3 = c->n3, b = sizeof(double);
   const int V = c->V, E = c->E;

@@ -3,6 +3,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "fft3d.h"
@@ -395,8 +398,37 @@ static int autotune_t(Fft3& f, hipStream_t s) {
   return rc;
 }
 
+// The plans chosen for a grid size and precision are kept for the process:
+// every later context of the same shape on the same device reuses them, so
+// contexts of one process (ensemble members and single runs alike) round the
+// same way, and the timing runs once per shape.
+namespace {
+struct Tuned {
+  int var[5];
+  float ms[5][3];
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, Tuned> g_tuned;
+}  // namespace
+
 int Fft3::autotune(hipStream_t s) {
-  return prec == MM_FP64 ? autotune_t<double>(*this, s) : autotune_t<float>(*this, s);
+  int dev = 0;
+  MM_HIP(hipGetDevice(&dev));
+  const auto key = std::make_tuple(nx, ny, nz, prec, dev);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) {
+    std::memcpy(var, it->second.var, sizeof(var));
+    std::memcpy(tune_ms, it->second.ms, sizeof(tune_ms));
+    return 0;
+  }
+  const int rc = prec == MM_FP64 ? autotune_t<double>(*this, s) : autotune_t<float>(*this, s);
+  if (rc) return rc;
+  Tuned t;
+  std::memcpy(t.var, var, sizeof(var));
+  std::memcpy(t.ms, tune_ms, sizeof(tune_ms));
+  g_tuned[key] = t;
+  return 0;
 }
 
 }  // namespace mm

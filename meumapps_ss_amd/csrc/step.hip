@@ -96,6 +96,46 @@ __global__ void k_gather(const Ctl* __restrict__ ctl, const double* __restrict__
 #define GRID_STRIDE(i, n) \
   for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < (n); i += (size_t)gridDim.x * kBlock)
 
+// The acoustic tensor Q(k) of every non-zero wave vector must be invertible:
+// eldis inverts it (invert(Q,3,3), ModElastic.f:187-189) and stops with
+// err_flag 3 when the reciprocal 1-norm condition number is <= 1e-16.  Q
+// depends only on CEla(:,:,2) and k, so a context checks it once, before its
+// first step that runs elasticity; the inverse used for the condition number
+// is the adjugate one of the step's acoustic solve.
+__global__ void k_qcheck(const DevParams* __restrict__ P, size_t nc, int* __restrict__ err) {
+  bool bad = false;
+  GRID_STRIDE(i, nc) {
+    const KVec k = kvec(P, i);
+    if (k.kx == 0 && k.ky == 0 && k.kz == 0) continue;
+    double Q[3][3];
+    acoustic_tensor(P, k, Q);
+    const double c00 = Q[1][1] * Q[2][2] - Q[1][2] * Q[2][1];
+    const double c01 = Q[1][2] * Q[2][0] - Q[1][0] * Q[2][2];
+    const double c02 = Q[1][0] * Q[2][1] - Q[1][1] * Q[2][0];
+    const double det = Q[0][0] * c00 + Q[0][1] * c01 + Q[0][2] * c02;
+    if (det == 0.0) { bad = true; continue; }
+    double A[3][3];
+    A[0][0] = c00; A[1][0] = c01; A[2][0] = c02;
+    A[0][1] = Q[0][2] * Q[2][1] - Q[0][1] * Q[2][2];
+    A[1][1] = Q[0][0] * Q[2][2] - Q[0][2] * Q[2][0];
+    A[2][1] = Q[0][1] * Q[2][0] - Q[0][0] * Q[2][1];
+    A[0][2] = Q[0][1] * Q[1][2] - Q[0][2] * Q[1][1];
+    A[1][2] = Q[0][2] * Q[1][0] - Q[0][0] * Q[1][2];
+    A[2][2] = Q[0][0] * Q[1][1] - Q[0][1] * Q[1][0];
+    double anorm = 0.0, inorm = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double a = fabs(Q[0][j]) + fabs(Q[1][j]) + fabs(Q[2][j]);
+      const double b = (fabs(A[0][j]) + fabs(A[1][j]) + fabs(A[2][j])) / fabs(det);
+      anorm = a > anorm ? a : anorm;
+      inorm = b > inorm ? b : inorm;
+    }
+    const double rcond = (anorm > 0.0 && inorm > 0.0) ? 1.0 / (anorm * inorm) : 0.0;
+    if (!(rcond > 1.0e-16)) bad = true;
+  }
+  if (bad) *err = MM_ERR_QINV;
+}
+
 // -------- Gaussian elimination, partial pivoting, N compile-time (KKS Zintf)
 template <int N>
 __device__ __forceinline__ bool solve_pivot(double (&A)[N][N], double (&b)[N]) {
@@ -1329,8 +1369,7 @@ struct mm_ctx {
   void* sstage = nullptr;
   // eager steps on the fused path run the Cahn-Hilliard update (which needs only
   // c and the fluxes of the chemistry) on a second stream, beside the elastic
-  // solve, with its own work spectra (4E) and stage (4); MM_CH_STREAM=0 keeps
-  // it on the main stream
+  // solve, with its own work spectra (4E) and stage (4), when MM_CH_STREAM=1
   bool ch_async = false;
   bool ch_launched = false;
   hipStream_t s2 = nullptr;
@@ -1372,6 +1411,7 @@ struct mm_ctx {
   void* noise_dev = nullptr;
   void* noise_pin = nullptr;
   int last_niter = 0;
+  bool qchecked = false;  // the acoustic tensor passed k_qcheck
   double last_cfac = 1.0;
   // graph mode: the regular step (step > elastep0) recorded once and replayed
   Ctl* ctl = nullptr;       // device control of the elastic iteration
@@ -1949,10 +1989,9 @@ template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
   MM_TRY(step_head<T>(c));
   // the Cahn-Hilliard update needs only c and the fluxes: on the second stream
   // it runs beside the elastic solve (joined before the Allen-Cahn update).
-  // While a probe times a kernel class other than the elastic z kernel the
-  // update stays on the main stream, so that the probed kernels run alone.
-  const int pk = c->probe.kind;
-  if (c->ch_async && (pk < 0 || pk == MM_PROBE_ZF_ELASTIC)) {
+  // While a probe times a kernel class the update stays on the main stream,
+  // so that the probed kernels run alone.
+  if (c->ch_async && c->probe.kind < 0) {
     C2<T>* w = (C2<T>*)c->sch;
     MM_HIP(hipEventRecord(c->ev_head, c->s));
     MM_HIP(hipStreamWaitEvent(c->s2, c->ev_head, 0));
@@ -2021,7 +2060,23 @@ template <typename T> int finish_replay(mm_ctx* c, const double* st, mm_diag* d)
 
 int graph_build(mm_ctx* c, hipStream_t cap);
 
+// Once per context, before the first step that runs elasticity (step >=
+// elastep0): the acoustic tensor check of k_qcheck (err_flag 3).
+int qcheck_once(mm_ctx* c) {
+  if (c->qchecked || c->step < c->prm.elastep0) return 0;
+  hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(kBlock), 0, c->s, c->derr, (size_t)1, 0, (const int*)nullptr);
+  hipLaunchKernelGGL(k_qcheck, dim3(c->nblk_k), dim3(kBlock), 0, c->s, c->dp, c->nc, c->derr);
+  MM_HIP(hipGetLastError());
+  int e = 0;
+  MM_HIP(hipMemcpyAsync(&e, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  MM_HIP(hipStreamSynchronize(c->s));
+  if (e) return e;
+  c->qchecked = true;
+  return 0;
+}
+
 template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
+  MM_TRY(qcheck_once(c));
   const bool regular = c->step > c->prm.elastep0 && c->step > c->prm.noise_step;
   if (!(c->graph && c->zfused && regular)) return run_step_eager<T>(c, d);
   if (!c->gx) MM_TRY(graph_build(c, c->s));
@@ -2313,11 +2368,12 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   if (!rc && hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) rc = MM_ERR_HIP;
   if (!rc && hipEventCreateWithFlags(&c->ev, hipEventDisableTiming) != hipSuccess) rc = MM_ERR_HIP;
   {
-    // measured: 1% faster at 256^3, 2% slower at 512^3 (its 512-thread elastic
-    // workgroups fill a CU each and lose CUs to the update), so by default up
-    // to 256^3 points; MM_CH_STREAM=0 / 1 switches it off / on for any size
+    // off by default: measured 1% faster at 256^3 but it slows the elastic
+    // kernels it runs beside (the polarization pass by 19%), and 2% slower at
+    // 512^3 (its 512-thread elastic workgroups fill a CU each and lose CUs to
+    // the update); MM_CH_STREAM=1 switches it on
     const char* e = std::getenv("MM_CH_STREAM");
-    const bool want = e && e[0] ? e[0] != '0' : n3 <= ((size_t)1 << 24);
+    const bool want = e && e[0] && e[0] != '0';
     c->ch_async = c->zfused && c->E <= MM_MAX_EL && want;
   }
   if (!rc && c->ch_async) {
@@ -2605,6 +2661,7 @@ extern "C" int mm_batch_step(mm_batch* b, int32_t nsteps, mm_diag* diag) {
   const size_t n = b->m.size();
   for (int k = 0; k < nsteps; ++k) {
     mm_diag* d = diag ? diag + (size_t)k * n : nullptr;
+    for (mm_ctx* c : b->m) MM_TRY(qcheck_once(c));
     MM_TRY(b->prec == MM_FP64 ? batch_step_one<double>(b, d) : batch_step_one<float>(b, d));
   }
   return 0;

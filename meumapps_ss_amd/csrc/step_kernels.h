@@ -190,11 +190,10 @@ __device__ __forceinline__ double kgk(const DevParams* __restrict__ P, int v, co
 struct Cplx3 {
   double re[3], im[3];
 };
-__device__ __forceinline__ void acoustic_solve(const DevParams* __restrict__ P, const KVec& k,
-                                               Cplx3& u) {
+__device__ __forceinline__ void acoustic_tensor(const DevParams* __restrict__ P, const KVec& k,
+                                                double (&Q)[3][3]) {
   const int vm[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
   const double* C = P->cela + 36;  // phase 2, column-major: C(a,b) = C[b*6+a]
-  double Q[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -209,6 +208,12 @@ __device__ __forceinline__ void acoustic_solve(const DevParams* __restrict__ P, 
         }
       Q[i][l] = s;
     }
+}
+
+__device__ __forceinline__ void acoustic_solve(const DevParams* __restrict__ P, const KVec& k,
+                                               Cplx3& u) {
+  double Q[3][3];
+  acoustic_tensor(P, k, Q);
   const double c00 = Q[1][1] * Q[2][2] - Q[1][2] * Q[2][1];
   const double c01 = Q[1][2] * Q[2][0] - Q[1][0] * Q[2][2];
   const double c02 = Q[1][0] * Q[2][1] - Q[1][1] * Q[2][0];

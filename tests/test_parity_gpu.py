@@ -17,6 +17,7 @@ from parity_cases import CASES, make_case
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
+LAGGED = ("glob_min", "glob_max", "phi_max", "solid_tot")
 
 
 def _rel(a, b):
@@ -70,11 +71,41 @@ def test_multi_step(gpu_lib, name, nsteps):
         err = np.abs(st[k] - ro[k]).max()
         assert err < TOL, f"{k} max-abs {err:.3e}"
     assert [r["n_elastic"] for r in rows] == [q["n_elastic"] for q in ref_rows]
+    # the fort.98 rows: energies, and the lagged check-loop columns
+    # (Multi_MEUMAPPS.f:905-906, from the check loop :1016-1062 of the step before)
+    for r, q in zip(rows, ref_rows):
+        for k in ("fint", "fch", "fw", "elen", "etot"):
+            assert abs(r[k] - q[k]) <= 1e-9 * max(1.0, abs(q[k])), (k, q["step"])
+        for k in LAGGED:
+            assert abs(r[k] - q[k]) < TOL, (k, q["step"])
     # Cahn-Hilliard conserves mass to round-off
     c0 = cons.mean(axis=(1, 2, 3))
     for r in rows:
         for e in range(p.numel):
             assert abs(r["caviter"][e] - c0[e]) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["v3_tetra", "v2_twophase"])
+def test_mixed_rows_match_oracle(gpu_lib, name):
+    """MIXED fort.98 / fort.99 rows against the FP64 oracle over 20 steps: the
+    lagged columns glob_min, glob_max, phi_max, solid_tot (formed from the FP64
+    mean composition plus the FP32 deviation, and FP32 phi) within 1e-5, the
+    energies within 1e-5 relative, the mean composition within 1e-9."""
+    p, phi, cons = make_case(name, seed=1)
+    o = Oracle(p, 4)
+    o.set_state(phi=phi, cons=cons)
+    ref_rows = o.step(20)
+    with Simulation(p, MM_MIXED) as s:
+        s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+        rows = s.step(20)
+    assert [r["n_elastic"] for r in rows] == [q["n_elastic"] for q in ref_rows]
+    for r, q in zip(rows, ref_rows):
+        for k in LAGGED:
+            assert abs(r[k] - q[k]) < 1e-5, (k, q["step"], r[k], q[k])
+        for k in ("fint", "fch", "fw", "elen", "etot"):
+            assert abs(r[k] - q[k]) <= 1e-5 * max(1.0, abs(q[k])), (k, q["step"])
+        for e in range(p.numel):
+            assert abs(r["caviter"][e] - q["caviter"][e]) < 1e-9
 
 
 def test_elastep0_later(gpu_lib):
