@@ -266,6 +266,21 @@ int mm_fft_forward(mm_fft *f, const void *real_dev, void *spec_dev, void *hip_st
 int mm_fft_inverse(mm_fft *f, void *spec_dev, void *real_dev, void *hip_stream);
 size_t mm_fft_spectrum_elems(const mm_fft *f);
 
+/* mm_fft_forward without a plan and without any allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The grid is given by (nx, ny, nz, precision) with the same
+ * limits as mm_fft_create: every axis 8..512 of the form 2^a 3^b, Nx/2 one of
+ * the x-pass lengths; other sizes return MM_ERR_ARG.  It runs the default
+ * plan of every pass (the plans mm_fft_create keeps with MM_FFT_VARIANT=0),
+ * so its spectra equal those of such a plan bit for bit.  The twiddle tables
+ * of every supported length live in a file-scope device array of 64 KiB per
+ * GPU (held until the process ends), filled once at the first call on each
+ * device (a synchronous host-to-device copy) and only read afterwards, so
+ * concurrent calls on one GPU do not interfere through it. */
+int mm_fft_forward_noalloc(int32_t nx, int32_t ny, int32_t nz, int32_t precision, const void *real_dev,
+                           void *spec_dev, void *hip_stream);
+
 /* ---- host helpers (host-side set-up; no device work) ---- */
 /* Numerical-Recipes ran2 as in ran_2.f:1-42, including the float32-folded
  * constants; state is kept in *state (35 int32) instead of SAVE storage. */

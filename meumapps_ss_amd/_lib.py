@@ -50,6 +50,7 @@ def load() -> ctypes.CDLL:
         "mm_fft_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
         "mm_fft_inverse": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
         "mm_fft_spectrum_elems": (c_size_t, [c_void_p]),
+        "mm_fft_forward_noalloc": (c_int32, [c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
         "mm_ran2": (c_double, [POINTER(c_int32), POINTER(c_int32)]),
         "mm_ran2_fill": (c_int32, [POINTER(c_int32), POINTER(c_int32), ctypes.c_int64, D]),
         "mm_langevin": (c_double, [POINTER(c_int32), POINTER(c_int32), c_double]),
@@ -94,7 +95,8 @@ def check(rc: int, where: str) -> None:
 EXPORTED = [
     "mm_create", "mm_destroy", "mm_upload", "mm_download", "mm_step", "mm_get_step",
     "mm_set_step", "mm_field", "mm_field_to_host", "mm_fft_create", "mm_fft_destroy",
-    "mm_fft_forward", "mm_fft_inverse", "mm_fft_spectrum_elems", "mm_ran2", "mm_ic_uniform",
+    "mm_fft_forward", "mm_fft_inverse", "mm_fft_spectrum_elems", "mm_fft_forward_noalloc", "mm_ran2",
+    "mm_ic_uniform",
     "mm_errstr", "mm_probe", "mm_probe_read", "mm_get_tuning", "mm_set_graph",
     "mm_batch_create", "mm_batch_member", "mm_batch_step", "mm_batch_destroy",
     "mm_adj_enable", "mm_adj_zero", "mm_adj_set", "mm_adj_get", "mm_state_bytes", "mm_state_save",

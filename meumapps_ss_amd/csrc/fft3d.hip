@@ -493,3 +493,105 @@ extern "C" int mm_fft_inverse(mm_fft* h, void* spec_dev, void* real_dev, void* s
   float* out[1] = {(float*)real_dev};
   return h->f.inverse<float>(sp, out, 1, s);
 }
+
+// ------------------------------------- forward transform that allocates nothing
+// mm_fft_forward_noalloc runs the passes of mm_fft_forward with the default
+// plans; the twiddle tables, which mm_fft_create allocates per plan, come from
+// `scratch`, a file-scope device array that holds the tables of every
+// supported length in both precisions (about 62 KiB).  The host never names
+// `scratch` in a kernel argument or a copy: at the first call on a device,
+// with that device current, hipGetSymbolAddress gives that device's address
+// of it, the tables are copied there once, and the address is kept per
+// device.  After that the array is only read, so no call clears or writes
+// it, and every call on a device reads the same tables.
+namespace {
+constexpr int kNLen[13] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+constexpr int kXLen[13] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+constexpr size_t sum_len(const int (&a)[13]) {
+  size_t s = 0;
+  for (int i = 0; i < 13; ++i) s += (size_t)a[i];
+  return s;
+}
+// complex entries over all tables; FP64 tables first, then the FP32 ones
+constexpr size_t kTwEntries = sum_len(kNLen) + sum_len(kXLen);
+constexpr size_t kScratchBytes = kTwEntries * 16 + kTwEntries * 8;
+constexpr size_t kScratchDoubles = 8192;  // 64 KiB
+static_assert(kScratchBytes <= kScratchDoubles * sizeof(double), "twiddle tables exceed scratch");
+constexpr int kMaxDevices = 64;
+
+// byte offset of the table of length n (xlen: the x-pass half length table)
+size_t tw_offset(int prec, bool xlen, int n) {
+  size_t off = prec == MM_FP64 ? 0 : kTwEntries * 16;
+  const size_t esz = prec == MM_FP64 ? 16 : 8;
+  for (int i = 0; i < 13; ++i) {
+    if (!xlen && kNLen[i] == n) return off;
+    off += (size_t)kNLen[i] * esz;
+  }
+  for (int i = 0; i < 13; ++i) {
+    if (xlen && kXLen[i] == n) return off;
+    off += (size_t)kXLen[i] * esz;
+  }
+  return (size_t)-1;
+}
+
+std::mutex g_na_mu;
+void* g_na_ptr[kMaxDevices] = {};
+double g_na_host[kScratchDoubles];  // host image of the tables, filled under g_na_mu
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// the device address of `scratch` on the current device, with the tables in it
+static int noalloc_scratch(char** base) {
+  int dev = 0;
+  MM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    MM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    double t[2 * 512];  // one table at a time
+    char* h = (char*)g_na_host;
+    for (int prec = 0; prec < 2; ++prec)
+      for (int x = 0; x < 2; ++x)
+        for (int i = 0; i < 13; ++i) {
+          const int n = x ? kXLen[i] : kNLen[i];
+          mm::make_twiddles(n, t);
+          char* dst = h + tw_offset(prec, x != 0, n);
+          if (prec == MM_FP64) {
+            std::memcpy(dst, t, 2 * (size_t)n * sizeof(double));
+          } else {
+            for (int e = 0; e < 2 * n; ++e) ((float*)dst)[e] = (float)t[e];
+          }
+        }
+    MM_HIP(hipMemcpy(p, g_na_host, kScratchBytes, hipMemcpyHostToDevice));
+    g_na_ptr[dev] = p;
+  }
+  *base = (char*)g_na_ptr[dev];
+  return 0;
+}
+
+extern "C" int mm_fft_forward_noalloc(int32_t nx, int32_t ny, int32_t nz, int32_t precision,
+                                      const void* real_dev, void* spec_dev, void* stream) {
+  if (!real_dev || !spec_dev) return MM_ERR_ARG;
+  if (!mm::fft_length_ok(nx) || !mm::fft_length_ok(ny) || !mm::fft_length_ok(nz) || !mm::xlen_ok(nx / 2))
+    return MM_ERR_ARG;
+  if (precision != MM_FP64 && precision != MM_MIXED) return MM_ERR_ARG;
+  char* base = nullptr;
+  MM_TRY(noalloc_scratch(&base));
+  mm::Fft3 f;  // a plan view over the tables in scratch; nothing to destroy
+  f.nx = nx; f.ny = ny; f.nz = nz; f.H = nx / 2 + 1; f.prec = precision;
+  f.twLx = base + tw_offset(precision, true, nx / 2);
+  f.twNx = base + tw_offset(precision, false, nx);
+  f.twNy = base + tw_offset(precision, false, ny);
+  f.twNz = base + tw_offset(precision, false, nz);
+  hipStream_t s = (hipStream_t)stream;
+  const double n3 = (double)(float)((long)nx * ny * nz);
+  void* out[1] = {spec_dev};
+  if (precision == MM_FP64) {
+    const double* in[1] = {(const double*)real_dev};
+    return f.forward<double>(in, out, 1, s, 1.0 / n3);
+  }
+  const float* in[1] = {(const float*)real_dev};
+  return f.forward<float>(in, out, 1, s, (float)(1.0 / n3));
+}
