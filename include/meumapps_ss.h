@@ -191,6 +191,28 @@ int mm_state_load(mm_ctx *ctx, const void *dev_src);
 int mm_adj_tape_step(mm_ctx *ctx, mm_diag *diag);
 int mm_adj_step(mm_ctx *ctx, mm_adj_grads *grads);
 
+/* Branch tape (no reference counterpart; for derivative checks).  The step
+ * takes two data-dependent branches per point and variant: the phi > 0 tests
+ * of the f_diff correction (Multi_MEUMAPPS.f:829-831) and the clamp of the
+ * check loop (:1034-1035).  Along a run the map from inputs to outputs is
+ * smooth only while those decisions stay fixed; the tape records them, counts
+ * how many a perturbed run changes, or replays them, so that finite
+ * differences can be taken on the frozen-branch map whose derivative the
+ * adjoint computes.  dev_buf is a caller-owned device buffer of nsteps *
+ * mm_branch_bytes() bytes (one byte per point and variant per step); each
+ * eager step uses the next slice.  mode: 0 off, 1 record, 2 count (the step
+ * decides as usual; decisions that differ from the tape are counted), 3
+ * replay (the tape's decisions are taken).  While a tape is set, steps run
+ * eagerly with the unfused clamp.  A step past nsteps returns MM_ERR_ARG.
+ * mm_branch_flips returns the count of mode 2 since mm_branch_tape. */
+#define MM_BRANCH_OFF 0
+#define MM_BRANCH_RECORD 1
+#define MM_BRANCH_COUNT 2
+#define MM_BRANCH_REPLAY 3
+int mm_branch_bytes(const mm_ctx *ctx, int64_t *bytes_per_step);
+int mm_branch_tape(mm_ctx *ctx, int32_t mode, void *dev_buf, int32_t nsteps);
+int mm_branch_flips(mm_ctx *ctx, int64_t *flips);
+
 /* Ensembles (no reference counterpart; the reference runs one simulation per
  * MPI job, Multi_MEUMAPPS.f:548-1106).  n independent members, each with its
  * own parameters and grid (params[0..n-1]) in one precision.  Fields go in

@@ -78,6 +78,9 @@ def load() -> ctypes.CDLL:
         "mm_state_load": (c_int32, [c_void_p, c_void_p]),
         "mm_adj_tape_step": (c_int32, [c_void_p, POINTER(MMDiag)]),
         "mm_adj_step": (c_int32, [c_void_p, POINTER(MMAdjGrads)]),
+        "mm_branch_bytes": (c_int32, [c_void_p, POINTER(ctypes.c_int64)]),
+        "mm_branch_tape": (c_int32, [c_void_p, c_int32, c_void_p, c_int32]),
+        "mm_branch_flips": (c_int32, [c_void_p, POINTER(ctypes.c_int64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -101,5 +104,6 @@ EXPORTED = [
     "mm_batch_create", "mm_batch_member", "mm_batch_step", "mm_batch_destroy",
     "mm_adj_enable", "mm_adj_zero", "mm_adj_set", "mm_adj_get", "mm_state_bytes", "mm_state_save",
     "mm_state_load", "mm_adj_tape_step", "mm_adj_step", "mm_langevin", "mm_set_noise",
-    "mm_get_noise_state", "mm_noise_accumulate", "mm_ran2_fill",
+    "mm_get_noise_state", "mm_noise_accumulate", "mm_ran2_fill", "mm_branch_bytes", "mm_branch_tape",
+    "mm_branch_flips",
 ]

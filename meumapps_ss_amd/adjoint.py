@@ -88,6 +88,24 @@ class AdjointRun:
         self.torch.cuda.synchronize()
         _lib.check(self.lib.mm_state_load(self.sim.ctx, ctypes.c_void_p(t.data_ptr())), "mm_state_load")
 
+    # -- branch tape (mm_branch_tape): record / count / replay the phi > 0
+    #    tests of the f_diff correction and the clamp, one byte per point and
+    #    variant per step, in a device buffer owned by torch
+    def new_branch_tape(self, nsteps: int):
+        nb = ctypes.c_int64()
+        _lib.check(self.lib.mm_branch_bytes(self.sim.ctx, ctypes.byref(nb)), "mm_branch_bytes")
+        return self.torch.zeros(nsteps * nb.value, dtype=self.torch.uint8, device="cuda")
+
+    def set_branch_tape(self, mode: int, buf=None, nsteps: int = 0):
+        ptr = None if buf is None else ctypes.c_void_p(buf.data_ptr())
+        self.torch.cuda.synchronize()
+        _lib.check(self.lib.mm_branch_tape(self.sim.ctx, mode, ptr, nsteps), "mm_branch_tape")
+
+    def branch_flips(self) -> int:
+        n = ctypes.c_int64()
+        _lib.check(self.lib.mm_branch_flips(self.sim.ctx, ctypes.byref(n)), "mm_branch_flips")
+        return n.value
+
     def start(self, phi0: np.ndarray, cons0: np.ndarray):
         self.sim.upload(phi=phi0, cons=cons0, u=np.zeros((3, *self.shape)), gradu=np.zeros((9, *self.shape)))
         self.sim.set_step(self.p.elastep0 + 1)

@@ -213,6 +213,17 @@ __device__ __forceinline__ double ppr(double f) {  // p'(phi), :656
   return 30.0 * fs * (fs - 2.0 * f + 1.0);
 }
 
+// Branch tape of one step (mm_branch_tape): one byte per point and variant,
+// bit 0 the phi > 0 test of the f_diff correction (Multi_MEUMAPPS.f:829-831),
+// bits 1-2 the clamp of the check loop (:1034-1035; 0 in [0, 1], 1 below 0,
+// 2 above 1).  mode 0 off, 1 record, 2 count (decide as usual, count the
+// decisions that differ from the tape), 3 replay (take the tape's decisions).
+struct BranchTape {
+  uint8_t* buf;
+  unsigned long long* flips;
+  int mode;
+};
+
 // ---------------------------------------------------------------- chemistry
 // Multi_MEUMAPPS.f:634-845: p, p', Pph, Gph, KKS interface compositions,
 // Gibbs energies, chemical potential, driving forces f_grad, fluxes F and
@@ -221,11 +232,12 @@ template <typename T, int V, int NPH, int E>
 __global__ void __launch_bounds__(kBlock)
 k_chem(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ gphi,
        const T* __restrict__ kdel, const T* __restrict__ cons, T* __restrict__ fdiff,
-       T* __restrict__ flux, double* __restrict__ part, int* __restrict__ err, size_t n3) {
+       T* __restrict__ flux, double* __restrict__ part, int* __restrict__ err, size_t n3, BranchTape bt) {
   constexpr int Z = NPH * E;
   __shared__ double sh[kWaves];
   double s_fint = 0.0, s_fch = 0.0, s_fw = 0.0;
   bool bad = false;
+  unsigned long long nflip = 0;
   GRID_STRIDE(pt, n3) {
     double f[V], gx[V], gy[V], gz[V], kd[V], c[E];
 #pragma unroll
@@ -329,14 +341,30 @@ k_chem(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __re
     s_fch += fch;
     s_fw += fw;
     // f_diff correction (:824-841)
+    bool pos[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) pos[v] = f[v] > 0.0;
+    if (bt.mode) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const size_t i = v * n3 + pt;
+        if (bt.mode == 1) {
+          bt.buf[i] = pos[v] ? 1 : 0;
+        } else {
+          const bool r = (bt.buf[i] & 1) != 0;
+          if (bt.mode == 2) nflip += (r != pos[v]);
+          else pos[v] = r;
+        }
+      }
+    }
 #pragma unroll
     for (int jv = 0; jv < V; ++jv) {
       double t11 = 0.0;
       int nt = 1;
-      if (f[jv] > 0.0) {
+      if (pos[jv]) {
 #pragma unroll
         for (int v = 0; v < V; ++v)
-          if (v != jv && f[v] > 0.0) { t11 = t11 + fg[v] - kd[v]; nt += 1; }
+          if (v != jv && pos[v]) { t11 = t11 + fg[v] - kd[v]; nt += 1; }
       }
       st(fdiff, jv * n3 + pt, fg[jv] - t11 / (double)nt);
     }
@@ -351,6 +379,7 @@ k_chem(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __re
   block_store<0>(s_fch, part, 1, sh);
   block_store<0>(s_fw, part, 2, sh);
   if (bad) *err = MM_ERR_ZSOLVE;
+  if (nflip) atomicAdd(bt.flips, nflip);
 }
 
 // p(phi) fields for the zeroth-order elastic solve (ModElastic.f:105-112)
@@ -960,10 +989,11 @@ k_elener(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __
 template <typename T, int V, int E>
 __global__ void __launch_bounds__(kBlock)
 k_clamp(const DevParams* __restrict__ P, T* __restrict__ phi, const T* __restrict__ cons,
-        double* __restrict__ part, size_t n3, const int* __restrict__ gate) {
+        double* __restrict__ part, size_t n3, const int* __restrict__ gate, BranchTape bt) {
   if (gated_off(gate)) return;
   __shared__ double sh[kWaves];
   double cs[E], cmin = HUGE_VAL, cmax = -HUGE_VAL, pmax = -HUGE_VAL, sf = 0.0;
+  unsigned long long nflip = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e) cs[e] = 0.0;
   GRID_STRIDE(pt, n3) {
@@ -978,8 +1008,20 @@ k_clamp(const DevParams* __restrict__ P, T* __restrict__ phi, const T* __restric
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       double f = ld(phi, v * n3 + pt);
-      if (f < 0.0) f = 0.0;
-      if (f > 1.0) f = 1.0;
+      int cl = f < 0.0 ? 1 : (f > 1.0 ? 2 : 0);
+      if (bt.mode) {
+        const size_t i = v * n3 + pt;
+        const uint8_t b = bt.buf[i];
+        if (bt.mode == 1) {
+          bt.buf[i] = (uint8_t)((b & 1) | (cl << 1));
+        } else {
+          const int r = (b >> 1) & 3;
+          if (bt.mode == 2) nflip += (r != cl);
+          else cl = r;
+        }
+      }
+      if (cl == 1) f = 0.0;
+      if (cl == 2) f = 1.0;
       st(phi, v * n3 + pt, f);
       f = (double)(T)f;
       if (f > pmax) pmax = f;
@@ -992,6 +1034,7 @@ k_clamp(const DevParams* __restrict__ P, T* __restrict__ phi, const T* __restric
   block_store<2>(cmax, part, E + 1, sh);
   block_store<2>(pmax, part, E + 2, sh);
   block_store<0>(sf, part, E + 3, sh);
+  if (nflip) atomicAdd(bt.flips, nflip);
 }
 
 // ============================================================ k-space kernels
@@ -1173,13 +1216,14 @@ k_ks_deriv(const DevParams* __restrict__ P, const C2<T>* __restrict__ in, C2<T>*
 
 // ============================================================ dispatch table
 template <typename T> struct Kern {
-  void (*chem)(dim3, hipStream_t, const DevParams*, const T*, const T*, const T*, const T*, T*, T*, double*, int*, size_t) = nullptr;
+  void (*chem)(dim3, hipStream_t, const DevParams*, const T*, const T*, const T*, const T*, T*, T*, double*, int*, size_t,
+               BranchTape) = nullptr;
   void (*ppoly)(dim3, hipStream_t, const T*, T*, size_t) = nullptr;
   void (*polar)(dim3, hipStream_t, const DevParams*, const T*, const T*, double, const double*, T*, int*, size_t,
                 const int*) = nullptr;
   void (*elener)(dim3, hipStream_t, const DevParams*, const T*, const T*, T*, T*, T*, double*, int*, size_t,
                  const int*) = nullptr;
-  void (*clamp)(dim3, hipStream_t, const DevParams*, T*, const T*, double*, size_t, const int*) = nullptr;
+  void (*clamp)(dim3, hipStream_t, const DevParams*, T*, const T*, double*, size_t, const int*, BranchTape) = nullptr;
   void (*zeroth)(dim3, hipStream_t, const DevParams*, const C2<T>*, C2<T>*, C2<T>*, size_t) = nullptr;
   // fused x pass + polarization stress (homogeneous moduli, supported x lengths only):
   // returns false when it has no kernel for this x length
@@ -1254,8 +1298,8 @@ static bool go_xclamp(bool launch, hipStream_t s, const DevParams* P, const C2<T
 
 template <typename T, int V, int NPH, int E> struct KernImpl {
   static void chem(dim3 g, hipStream_t s, const DevParams* P, const T* a, const T* b, const T* c, const T* d,
-                   T* e, T* f, double* p, int* err, size_t n) {
-    hipLaunchKernelGGL((k_chem<T, V, NPH, E>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, f, p, err, n);
+                   T* e, T* f, double* p, int* err, size_t n, BranchTape bt) {
+    hipLaunchKernelGGL((k_chem<T, V, NPH, E>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, f, p, err, n, bt);
   }
   static void ppoly(dim3 g, hipStream_t s, const T* a, T* b, size_t n) {
     hipLaunchKernelGGL((k_ppoly<T, V>), g, dim3(kBlock), 0, s, a, b, n);
@@ -1271,8 +1315,8 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
     hipLaunchKernelGGL((k_elener<T, V, NPH, H>), g, dim3(kBlock), 0, s, P, a, b, c, d, e, p, err, n, gate);
   }
   static void clamp(dim3 g, hipStream_t s, const DevParams* P, T* a, const T* b, double* p, size_t n,
-                    const int* gate) {
-    hipLaunchKernelGGL((k_clamp<T, V, E>), g, dim3(kBlock), 0, s, P, a, b, p, n, gate);
+                    const int* gate, BranchTape bt) {
+    hipLaunchKernelGGL((k_clamp<T, V, E>), g, dim3(kBlock), 0, s, P, a, b, p, n, gate, bt);
   }
   static void zeroth(dim3 g, hipStream_t s, const DevParams* P, const C2<T>* a, C2<T>* b, C2<T>* c, size_t n) {
     hipLaunchKernelGGL((k_ks_zeroth<T, V>), g, dim3(kBlock), 0, s, P, a, b, c, n);
@@ -1412,6 +1456,13 @@ struct mm_ctx {
   void* noise_pin = nullptr;
   int last_niter = 0;
   bool qchecked = false;  // the acoustic tensor passed k_qcheck
+  // branch tape (mm_branch_tape): caller's device buffer of br_cap steps,
+  // br_stride bytes each; bt_now is the tape of the step being run (mode 0
+  // outside run_step_eager)
+  BranchTape br{nullptr, nullptr, 0};
+  BranchTape bt_now{nullptr, nullptr, 0};
+  int64_t br_cap = 0, br_cursor = 0;
+  unsigned long long* dflips = nullptr;
   double last_cfac = 1.0;
   // graph mode: the regular step (step > elastep0) recorded once and replayed
   Ctl* ctl = nullptr;       // device control of the elastic iteration
@@ -1775,7 +1826,7 @@ template <typename T> int step_head(mm_ctx* c) {
   // 3. chemistry (:634-845)
   c->probe.begin(MM_PROBE_CHEM, c->s);
   K.chem(dim3(c->nblk_pw), c->s, c->dp, phi, (T*)c->gphi, (T*)c->kdel, (T*)c->cons, (T*)c->fdiff, (T*)c->flux,
-         c->part, c->derr, n3);
+         c->part, c->derr, n3, c->bt_now);
   c->probe.end(MM_PROBE_CHEM, c->s, (double)n3 * sizeof(T) * (6 * V + 4 * E));
   MM_HIP(hipGetLastError());
   MM_TRY(finalize(c, c->nblk_pw, 3, {0, 0, 0}, R_FINT));
@@ -1892,7 +1943,7 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
       opac.gate = c->gate;
       MM_TRY(launch_zfused<T>(c->fft, opac, V, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 3 * V, nullptr));
       for (int v = 0; v < V; ++v) { sp[v] = work + (size_t)v * nc; rp[v] = phi + (size_t)v * n3; }
-      if (c->xclamp && !rec_phipre) {
+      if (c->xclamp && !rec_phipre && !c->bt_now.mode) {
         // inverse y pass into the stage; the x pass, the clamp and the
         // check-loop sums in one kernel (phi written once)
         C2<T>* stg = (C2<T>*)c->sstage;
@@ -1916,7 +1967,7 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   }
   // clamp and check-loop sums (:1008-1065)
   if (rec_phipre) MM_HIP(hipMemcpyAsync(rec_phipre, phi, (size_t)V * n3 * sizeof(T), hipMemcpyDeviceToDevice, c->s));
-  K.clamp(gp, c->s, c->dp, phi, (T*)c->cons, c->part, n3, c->gate);
+  K.clamp(gp, c->s, c->dp, phi, (T*)c->cons, c->part, n3, c->gate, c->bt_now);
   MM_HIP(hipGetLastError());
   return finalize_clamp(c, c->nblk_pw);
 }
@@ -1977,7 +2028,19 @@ template <typename T> int noise_advance(mm_ctx* c) {
   return 0;
 }
 
+template <typename T> int run_step_eager_body(mm_ctx* c, mm_diag* d);
 template <typename T> int run_step_eager(mm_ctx* c, mm_diag* d) {
+  if (!c->br.mode) return run_step_eager_body<T>(c, d);
+  if (c->br_cursor >= c->br_cap) return MM_ERR_ARG;  // the tape is full
+  const size_t stride = (size_t)c->V * c->n3;
+  c->bt_now = BranchTape{c->br.buf + (size_t)c->br_cursor * stride, c->dflips, c->br.mode};
+  const int rc = run_step_eager_body<T>(c, d);
+  c->bt_now = BranchTape{nullptr, nullptr, 0};
+  if (!rc) c->br_cursor += 1;
+  return rc;
+}
+
+template <typename T> int run_step_eager_body(mm_ctx* c, mm_diag* d) {
   const bool noisy = c->step <= c->prm.noise_step;
   if (noisy && !c->noise_on) return MM_ERR_ARG;  // mm_set_noise first
   if (!noisy && c->noise_live) {
@@ -2078,7 +2141,7 @@ int qcheck_once(mm_ctx* c) {
 template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
   MM_TRY(qcheck_once(c));
   const bool regular = c->step > c->prm.elastep0 && c->step > c->prm.noise_step;
-  if (!(c->graph && c->zfused && regular)) return run_step_eager<T>(c, d);
+  if (!(c->graph && c->zfused && regular) || c->br.mode) return run_step_eager<T>(c, d);
   if (!c->gx) MM_TRY(graph_build(c, c->s));
   MM_HIP(hipGraphLaunch(c->gx, c->s));
   MM_HIP(hipMemcpyAsync(c->hstage, c->dstage, kStage * sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -2110,7 +2173,7 @@ void free_all(mm_ctx* c) {
   c->g = nullptr;
   void* bufs[] = {c->phi, c->cons, c->gphi, c->kdel, c->fdiff, c->flux, c->strain, c->xpol, c->el_en,
                   c->elgrad, c->sphi, c->suhat, c->swork, c->part, c->dres, c->derr, c->dp, c->ctl,
-                  c->dstage, c->sstage, c->sstrain};
+                  c->dstage, c->sstage, c->sstrain, c->dflips};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
@@ -2422,6 +2485,35 @@ extern "C" int mm_step(mm_ctx* c, int32_t nsteps, mm_diag* diag) {
                                 : run_step<float>(c, diag ? diag + i : nullptr);
     if (rc) return rc;
   }
+  return 0;
+}
+
+extern "C" int mm_branch_bytes(const mm_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return MM_ERR_ARG;
+  *bytes = (int64_t)c->V * (int64_t)c->n3;
+  return 0;
+}
+
+extern "C" int mm_branch_tape(mm_ctx* c, int32_t mode, void* dev_buf, int32_t nsteps) {
+  if (!c || mode < 0 || mode > 3) return MM_ERR_ARG;
+  if (mode && (!dev_buf || nsteps <= 0)) return MM_ERR_ARG;
+  if (mode && !c->dflips) MM_TRY(dalloc((void**)&c->dflips, sizeof(unsigned long long)));
+  if (c->dflips) MM_HIP(hipMemsetAsync(c->dflips, 0, sizeof(unsigned long long), c->s));
+  MM_HIP(hipStreamSynchronize(c->s));
+  c->br = BranchTape{(uint8_t*)dev_buf, c->dflips, mode};
+  c->br_cap = mode ? nsteps : 0;
+  c->br_cursor = 0;
+  return 0;
+}
+
+extern "C" int mm_branch_flips(mm_ctx* c, int64_t* flips) {
+  if (!c || !flips) return MM_ERR_ARG;
+  unsigned long long h = 0;
+  if (c->dflips) {
+    MM_HIP(hipMemcpyAsync(&h, c->dflips, sizeof(h), hipMemcpyDeviceToHost, c->s));
+    MM_HIP(hipStreamSynchronize(c->s));
+  }
+  *flips = (int64_t)h;
   return 0;
 }
 
