@@ -164,7 +164,10 @@ int mm_set_graph(mm_ctx *ctx, int32_t on, int32_t iters);
  *                      path and graph mode are switched off on this context)
  *   mm_state_bytes / mm_state_save / mm_state_load
  *                      device-memory record of the dynamic state (phi, c, strain,
- *                      u_hat, step and lagged check-loop values): checkpoints
+ *                      u_hat, step, lagged check-loop values and, in MM_MIXED, the
+ *                      FP64 mean composition the stored FP32 deviation is taken
+ *                      from): checkpoints; a record loads into any context of the
+ *                      same model and precision (MM_ERR_ARG for another E)
  *   mm_adj_tape_step   one forward step from the current state, recording what
  *                      its adjoint needs (state advances as in mm_step)
  *   mm_adj_step        pull the adjoint state back through the taped step and
@@ -232,6 +235,9 @@ void mm_batch_destroy(mm_batch *batch);
 
 /* Device-resident field access for in-process callers (torch, tests):
  * returns the device address of one SoA field and its element size (8 or 4).
+ * In MM_MIXED the device composition field (which = 1) holds the FP32
+ * deviation c - cbar from the FP64 mean cbar of the last upload;
+ * mm_field_to_host adds cbar back.
  * which: 0 phi, 1 cons, 2 el_en, 3 f_diff (last step), 4 strain (6 fields:
  * e11, e22, e33, 2e23, 2e13, 2e12 of the displacement u; e_mean is not
  * included), 5 grad_phi (3V fields),

@@ -763,16 +763,21 @@ extern "C" int mm_adj_get(mm_ctx* c, double* lphi, double* lcons, double* leps) 
 // ------------------------------------------------------------ state records
 // [header 8 doubles: step, glob_min, glob_max, phi_max, solid_frac, 0, 0, 0]
 // phi (V n3), cons (E n3), strain (6 n3), u_hat (3 nc complex)
+constexpr int kStateHead = 6 + MM_MAX_EL + 2;  // doubles of the state record header
+
 extern "C" int mm_state_bytes(const mm_ctx* c, int64_t* bytes) {
   if (!c || !bytes) return MM_ERR_ARG;
-  *bytes = (int64_t)(8 * sizeof(double) + (c->V + c->E + 6) * c->n3 * c->esz + 3 * c->nc * 2 * c->esz);
+  *bytes = (int64_t)(kStateHead * sizeof(double) + (c->V + c->E + 6) * c->n3 * c->esz + 3 * c->nc * 2 * c->esz);
   return 0;
 }
 
 extern "C" int mm_state_save(mm_ctx* c, void* dst) {
   if (!c || !dst) return MM_ERR_ARG;
   const size_t n3 = c->n3, es = c->esz;
-  const double h[8] = {(double)c->step, c->glob_min, c->glob_max, c->phi_max, c->solid_frac, 0, 0, 0};
+  // header: step, the lagged check-loop values, E and the mean composition
+  // cbar (MIXED stores c - cbar; 0 in FP64)
+  double h[kStateHead] = {(double)c->step, c->glob_min, c->glob_max, c->phi_max, c->solid_frac, (double)c->E};
+  for (int e = 0; e < c->E; ++e) h[6 + e] = c->hp.cbar[e];
   char* p = (char*)dst;
   MM_HIP(hipStreamSynchronize(c->s));
   MM_HIP(hipMemcpy(p, h, sizeof(h), hipMemcpyHostToDevice));
@@ -789,10 +794,11 @@ extern "C" int mm_state_save(mm_ctx* c, void* dst) {
 extern "C" int mm_state_load(mm_ctx* c, const void* src) {
   if (!c || !src) return MM_ERR_ARG;
   const size_t n3 = c->n3, es = c->esz;
-  double h[8];
+  double h[kStateHead];
   const char* p = (const char*)src;
   MM_HIP(hipStreamSynchronize(c->s));
   MM_HIP(hipMemcpy(h, p, sizeof(h), hipMemcpyDeviceToHost));
+  if ((int)h[5] != c->E) return MM_ERR_ARG;  // a record of another model
   p += sizeof(h);
   MM_TRY(d2d(c, c->phi, p, c->V * n3 * es)); p += c->V * n3 * es;
   MM_TRY(d2d(c, c->cons, p, c->E * n3 * es)); p += c->E * n3 * es;
@@ -802,6 +808,12 @@ extern "C" int mm_state_load(mm_ctx* c, const void* src) {
   MM_HIP(hipStreamSynchronize(c->s));
   c->step = (int)h[0];
   c->glob_min = h[1]; c->glob_max = h[2]; c->phi_max = h[3]; c->solid_frac = h[4];
+  bool same = true;
+  for (int e = 0; e < c->E; ++e) same = same && c->hp.cbar[e] == h[6 + e];
+  if (!same) {
+    for (int e = 0; e < c->E; ++e) c->hp.cbar[e] = h[6 + e];
+    MM_HIP(hipMemcpy(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice));
+  }
   if (c->adj) c->adj->taped = false;
   return 0;
 }

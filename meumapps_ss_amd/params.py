@@ -407,9 +407,29 @@ def read_input_txt(path: str, sym_var_path: Optional[str] = None) -> Inputs:
         cand = os.path.join(os.path.dirname(os.path.abspath(path)), "sym_var.inp")
         sym_var_path = cand if os.path.exists(cand) else None
     if sym_var_path:
-        vals = [_num(x) for x in _tokens(open(sym_var_path).read().replace("\n", " "))]
-        inp.sym_var = np.array(vals[: 12 * 9]).reshape(12, 3, 3)
+        inp.sym_var = read_sym_var(sym_var_path)
     return inp
+
+
+def read_sym_var(path: str, nvar: int = 12) -> np.ndarray:
+    """The variant rotations of sym_var.inp as InitTransGrad reads them
+    (ModInitials.f:307-310): per variant three list-directed reads
+    ``read(21,*) (SYM(i,j), j=1,3)``, one per row i.  Each read starts at a new
+    record, takes three values (continuing onto the next records if one holds
+    fewer) and discards the rest of the record it ends in."""
+    recs = [_tokens(l) for l in open(path).read().splitlines()]
+    out = np.zeros((nvar, 3, 3))
+    r = 0
+    for v in range(nvar):
+        for i in range(3):
+            vals = []
+            while len(vals) < 3:
+                if r >= len(recs):
+                    raise SetupError(2, "Check input inp file: Initial error 2")
+                vals += recs[r]
+                r += 1
+            out[v, i] = [_num(x) for x in vals[:3]]
+    return out
 
 
 def write_input_txt(inp: Inputs, path: str) -> None:

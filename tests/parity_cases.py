@@ -44,6 +44,19 @@ CASES = {
     # nv = 12 from sym_var rotations (InitTransGrad, ModInitials.f:200-340): an
     # orthorhombic eigenstrain turned about z in steps of 15 degrees
     "v12_rot": (16, 12, (0.02, -0.01, 0.005), 0.005, {"sym_var": "z15"}, "rich"),
+    # non-cubic precipitate moduli completed by eltensor (ModInitials.f:16-72):
+    # tetragonal (Ctype 2), hexagonal (Ctype 3) and trigonal (Ctype 4) against a
+    # cubic matrix, so the moduli are inhomogeneous and the acoustic tensor of
+    # the precipitate phase is not cubic
+    "v1_tetC": (16, 1, (0.01, 0.012, 0.006), 0.005,
+                {"ctype2": 2, "cela2": {(1, 1): 260.0, (1, 2): 140.0, (1, 3): 120.0, (3, 3): 230.0,
+                                        (4, 4): 90.0, (6, 6): 110.0}}, "rich"),
+    "v1_hexC": (16, 1, (0.01, 0.012, 0.006), 0.005,
+                {"ctype2": 3, "cela2": {(1, 1): 260.0, (1, 2): 140.0, (1, 3): 120.0, (3, 3): 230.0,
+                                        (4, 4): 90.0}}, "rich"),
+    "v1_trigC": (16, 1, (0.01, 0.012, 0.006), 0.005,
+                 {"ctype2": 4, "cela2": {(1, 1): 260.0, (1, 2): 140.0, (1, 3): 120.0, (3, 3): 230.0,
+                                         (4, 4): 90.0, (1, 5): 15.0}}, "rich"),
 }
 
 

@@ -56,7 +56,7 @@ def test_first_step_intermediates(gpu_lib, name):
 
 @pytest.mark.parametrize("name,nsteps", [("v3_tetra", 40), ("v3_inhomog", 20), ("v2_twophase", 30),
                                          ("v3_noise", 60), ("v1_e2", 20), ("v12_rot", 10),
-                                         ("v3_n24", 20)])
+                                         ("v3_n24", 20), ("v1_tetC", 10), ("v1_hexC", 10), ("v1_trigC", 10)])
 def test_multi_step(gpu_lib, name, nsteps):
     p, phi, cons = make_case(name, seed=1)
     o = Oracle(p, 4)
@@ -106,6 +106,39 @@ def test_mixed_rows_match_oracle(gpu_lib, name):
             assert abs(r[k] - q[k]) <= 1e-5 * max(1.0, abs(q[k])), (k, q["step"])
         for e in range(p.numel):
             assert abs(r["caviter"][e] - q["caviter"][e]) < 1e-9
+
+
+def test_nv12_model_read_from_input_files(gpu_lib, tmp_path):
+    """The nv = 12 model as a run sets it up: Input.txt and sym_var.inp written
+    to disk, read back by read_input_txt (the rotations record by record, as
+    InitTransGrad reads them, ModInitials.f:200-340), set up, and stepped on
+    the GPU against the oracle."""
+    from meumapps_ss_amd.params import read_input_txt, setup, write_input_txt
+    from parity_cases import z_rotations
+
+    from meumapps_ss_amd.configs import model_inputs
+
+    p0, phi, cons = make_case("v12_rot", seed=2)
+    inp = model_inputs(16, 12, (0.02, -0.01, 0.005), 0.005)
+    write_input_txt(inp, str(tmp_path / "Input.txt"))
+    with open(tmp_path / "sym_var.inp", "w") as f:
+        for s_ in z_rotations():
+            for row in s_:
+                f.write(" ".join(repr(float(x)) for x in row) + "\n")
+    p = setup(read_input_txt(str(tmp_path / "Input.txt")))
+    assert np.array_equal(p.trans_ppt_array(), p0.trans_ppt_array())
+    assert np.array_equal(np.ctypeslib.as_array(p.grad_coeff), np.ctypeslib.as_array(p0.grad_coeff))
+    o = Oracle(p, 4)
+    o.set_state(phi=phi, cons=cons)
+    ref_rows = o.step(4)
+    with Simulation(p, MM_FP64) as s:
+        s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+        rows = s.step(4)
+        st = s.download()
+    ro = o.get_state()
+    assert [r["n_elastic"] for r in rows] == [q["n_elastic"] for q in ref_rows]
+    for k in ("phi", "cons"):
+        assert np.abs(st[k] - ro[k]).max() < TOL, k
 
 
 def test_elastep0_later(gpu_lib):
@@ -253,6 +286,45 @@ def test_state_save_load_roundtrip(gpu_lib):
     for k in ("phi", "cons"):
         assert np.abs(a[k] - b[k]).max() < 1e-12, k
     assert _rel(a["u"], b["u"]) < 1e-12
+
+
+def test_mixed_state_load_restores_the_mean_composition(gpu_lib):
+    """MIXED stores c - cbar with cbar the FP64 mean of the upload.  A state
+    saved in one context and loaded after an upload of other fields (another
+    cbar), or into a second context, steps exactly like the uninterrupted
+    run: the record carries cbar."""
+    import ctypes
+
+    import torch
+    from meumapps_ss_amd import _lib as L
+    p, phi, cons = make_case("v3_tetra", seed=7)
+    zeros = dict(u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+    with Simulation(p, MM_MIXED) as s, Simulation(p, MM_MIXED) as t:
+        lib = s.lib
+        s.upload(phi=phi, cons=cons, **zeros)
+        s.step(3)
+        nb = ctypes.c_int64()
+        L.check(lib.mm_state_bytes(s.ctx, ctypes.byref(nb)), "mm_state_bytes")
+        buf = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+        L.check(lib.mm_state_save(s.ctx, ctypes.c_void_p(buf.data_ptr())), "mm_state_save")
+        s.step(3)
+        a = s.download()
+        rows_a = s.step(1)
+        # other fields, another mean composition, then the saved state
+        s.upload(phi=0.5 * phi, cons=cons + 0.05, **zeros)
+        L.check(lib.mm_state_load(s.ctx, ctypes.c_void_p(buf.data_ptr())), "mm_state_load")
+        s.step(3)
+        b = s.download()
+        rows_b = s.step(1)
+        # a second context that never saw the original upload
+        t.upload(phi=phi, cons=cons + 0.1, **zeros)
+        L.check(lib.mm_state_load(t.ctx, ctypes.c_void_p(buf.data_ptr())), "mm_state_load")
+        t.step(3)
+        c = t.download()
+    for k in ("phi", "cons"):
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], c[k]), k
+    assert rows_a[0]["caviter"] == rows_b[0]["caviter"]
 
 
 def test_backtracking_matches_oracle(gpu_lib):
