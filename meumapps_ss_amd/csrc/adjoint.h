@@ -723,6 +723,7 @@ extern "C" int mm_adj_enable(mm_ctx* c) {
   rc = strain_to_real_any(c);
   if (rc) { adj_release(a); return rc; }
   c->xpolar = false;
+  c->xchem = false;  // the tape copies grad phi and del.G del phi after the head
   if (c->gx) { (void)hipGraphExecDestroy(c->gx); c->gx = nullptr; }
   if (c->g) { (void)hipGraphDestroy(c->g); c->g = nullptr; }
   c->graph = false;

@@ -225,6 +225,119 @@ struct BranchTape {
 };
 
 // ---------------------------------------------------------------- chemistry
+// KKS chemistry at one point (Multi_MEUMAPPS.f:634-803): p, p', Pph, Gph, the
+// interface compositions (Zintf x = Zr, :677-710), Gibbs energies, chemical
+// potential; returns the driving forces fg (before the f_diff correction),
+// the flux coefficients cf[v][e] = p'(phi_v)(x_{e,1} - x_{e,p(v)}), so that
+// F_e = sum_v cf[v][e] grad phi_v in variant order, and the energy densities
+// fch, fw.  false: Zintf singular.
+template <int V, int NPH, int E>
+__device__ __forceinline__ bool chem_point(const DevParams* __restrict__ P, const double (&f)[V], const double (&c)[E],
+                                           double (&fg)[V], double (&cf)[V][E], double& fch_out, double& fw_out) {
+  constexpr int Z = NPH * E;
+  double Ppr[V], Pph[NPH + 1], Gph[NPH + 1];
+#pragma unroll
+  for (int q = 0; q <= NPH; ++q) { Pph[q] = 0.0; Gph[q] = 0.0; }
+#pragma unroll
+  for (int q = 2; q <= NPH; ++q)
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (P->phase_of[v] == q) {
+        const double fi = f[v], fs = fi * fi;
+        const double pl = fs * fi * (6.0 * fs - 15.0 * fi + 10.0);
+        Ppr[v] = 30.0 * fs * (fs - 2.0 * fi + 1.0);
+        Pph[1] += pl;
+        Pph[q] += pl;
+        Gph[q] += fs * ((1.0 - fi) * (1.0 - fi));
+#pragma unroll
+        for (int jv = 0; jv < V; ++jv)
+          if (jv != v) Gph[1] += fs * (f[jv] * f[jv]);
+      }
+  Pph[1] = 1.0 - Pph[1];
+  // Zintf x = Zr (constant rows :426-448, mixture rows :677-688)
+  double A[Z][Z], b[Z];
+#pragma unroll
+  for (int i = 0; i < Z; ++i) {
+#pragma unroll
+    for (int j = 0; j < Z; ++j) A[i][j] = P->zc[i * 16 + j];
+    b[i] = P->zrc[i];
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i0 = Z - E + e;
+#pragma unroll
+    for (int ph = 0; ph < NPH; ++ph) A[i0][NPH * e + ph] = Pph[ph + 1];
+    b[i0] = c[e];
+  }
+  const bool ok = solve_pivot<Z>(A, b);
+  // b[NPH*e + ph] = Xintf(e+1, ph+1)
+  const double* gp = P->gp;
+#define GP(j, ph) gp[((ph) - 1) * MM_GP_STRIDE + ((j) - 1)]
+  double Gf[NPH + 1], Mu[E];
+#pragma unroll
+  for (int ph = 1; ph <= NPH; ++ph) {
+    double g = 0.0;
+#pragma unroll
+    for (int e = 1; e <= E; ++e) {
+      const double dx = b[NPH * (e - 1) + ph - 1] - GP(e + E + 1, ph);
+      g += GP(e, ph) * (dx * dx);
+    }
+    Gf[ph] = g + GP(E + 1, ph);
+  }
+  Gf[0] = 0.0;
+#pragma unroll
+  for (int e = 1; e <= E; ++e) Mu[e - 1] = 2.0 * GP(e, 1) * (b[NPH * (e - 1)] - GP(e + E + 1, 1));
+  double fch = Pph[1] * Gf[1];
+  double fw = P->well[0] * Gph[1];
+#pragma unroll
+  for (int q = 2; q <= NPH; ++q) {
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (P->phase_of[v] == q) {
+        const double fi = f[v];
+        double termch = Gf[q] - Gf[1];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double xd = b[NPH * e] - b[NPH * e + q - 1];
+          termch = termch + Mu[e] * xd;
+          cf[v][e] = Ppr[v] * xd;
+        }
+        double wg = 2.0 * P->well[q - 1] * fi * (1.0 - fi) * (1.0 - 2.0 * fi);
+#pragma unroll
+        for (int jv = 0; jv < V; ++jv)
+          if (jv != v) wg = wg + 4.0 * P->well[0] * fi * (f[jv] * f[jv]);
+        fg[v] = termch * Ppr[v] + wg;
+      }
+    fch = fch + Pph[q] * Gf[q];
+    fw = fw + P->well[q - 1] * Gph[q];
+  }
+#undef GP
+  fch_out = fch;
+  fw_out = fw;
+  return ok;
+}
+
+// gradient energy density 1/2 grad phi . G grad phi of variant v (:768-775)
+__device__ __forceinline__ double grad_term(const DevParams* __restrict__ P, int v, double gx, double gy, double gz) {
+  const double* g = P->gc + v * 9;
+  return g[0] * (gx * gx) + g[4] * (gy * gy) + g[8] * (gz * gz) + (g[3] + g[1]) * gx * gy +
+         (g[6] + g[2]) * gx * gz + (g[7] + g[5]) * gy * gz;
+}
+
+// f_diff correction (:824-841) from the driving forces, del.G del phi and the
+// phi > 0 decisions
+template <int V>
+__device__ __forceinline__ double fdiff_corr(int jv, const double (&fg)[V], const double (&kd)[V], const bool (&pos)[V]) {
+  double t11 = 0.0;
+  int nt = 1;
+  if (pos[jv]) {
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (v != jv && pos[v]) { t11 = t11 + fg[v] - kd[v]; nt += 1; }
+  }
+  return fg[jv] - t11 / (double)nt;
+}
+
 // Multi_MEUMAPPS.f:634-845: p, p', Pph, Gph, KKS interface compositions,
 // Gibbs energies, chemical potential, driving forces f_grad, fluxes F and
 // the f_diff correction.  Partial sums: f_int, fch, fw.
@@ -233,7 +346,6 @@ __global__ void __launch_bounds__(kBlock)
 k_chem(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __restrict__ gphi,
        const T* __restrict__ kdel, const T* __restrict__ cons, T* __restrict__ fdiff,
        T* __restrict__ flux, double* __restrict__ part, int* __restrict__ err, size_t n3, BranchTape bt) {
-  constexpr int Z = NPH * E;
   __shared__ double sh[kWaves];
   double s_fint = 0.0, s_fch = 0.0, s_fw = 0.0;
   bool bad = false;
@@ -250,94 +362,21 @@ k_chem(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __re
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) c[e] = P->cbar[e] + ld(cons, e * n3 + pt);
-    double Ppr[V], Pph[NPH + 1], Gph[NPH + 1];
-#pragma unroll
-    for (int q = 0; q <= NPH; ++q) { Pph[q] = 0.0; Gph[q] = 0.0; }
-#pragma unroll
-    for (int q = 2; q <= NPH; ++q)
-#pragma unroll
-      for (int v = 0; v < V; ++v)
-        if (P->phase_of[v] == q) {
-          const double fi = f[v], fs = fi * fi;
-          const double pl = fs * fi * (6.0 * fs - 15.0 * fi + 10.0);
-          Ppr[v] = 30.0 * fs * (fs - 2.0 * fi + 1.0);
-          Pph[1] += pl;
-          Pph[q] += pl;
-          Gph[q] += fs * ((1.0 - fi) * (1.0 - fi));
-#pragma unroll
-          for (int jv = 0; jv < V; ++jv)
-            if (jv != v) Gph[1] += fs * (f[jv] * f[jv]);
-        }
-    Pph[1] = 1.0 - Pph[1];
-    // Zintf x = Zr (constant rows :426-448, mixture rows :677-688)
-    double A[Z][Z], b[Z];
-#pragma unroll
-    for (int i = 0; i < Z; ++i) {
-#pragma unroll
-      for (int j = 0; j < Z; ++j) A[i][j] = P->zc[i * 16 + j];
-      b[i] = P->zrc[i];
-    }
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i0 = Z - E + e;
-#pragma unroll
-      for (int ph = 0; ph < NPH; ++ph) A[i0][NPH * e + ph] = Pph[ph + 1];
-      b[i0] = c[e];
-    }
-    if (!solve_pivot<Z>(A, b)) bad = true;
-    // b[NPH*e + ph] = Xintf(e+1, ph+1)
-    const double* gp = P->gp;
-#define GP(j, ph) gp[((ph) - 1) * MM_GP_STRIDE + ((j) - 1)]
-    double Gf[NPH + 1], Mu[E];
-#pragma unroll
-    for (int ph = 1; ph <= NPH; ++ph) {
-      double g = 0.0;
-#pragma unroll
-      for (int e = 1; e <= E; ++e) {
-        const double dx = b[NPH * (e - 1) + ph - 1] - GP(e + E + 1, ph);
-        g += GP(e, ph) * (dx * dx);
-      }
-      Gf[ph] = g + GP(E + 1, ph);
-    }
-    Gf[0] = 0.0;
-#pragma unroll
-    for (int e = 1; e <= E; ++e) Mu[e - 1] = 2.0 * GP(e, 1) * (b[NPH * (e - 1)] - GP(e + E + 1, 1));
-    double fch = Pph[1] * Gf[1];
-    double fw = P->well[0] * Gph[1];
-    double Fx[E], Fy[E], Fz[E], fg[V];
+    double fg[V], cf[V][E], fch, fw;
+    if (!chem_point<V, NPH, E>(P, f, c, fg, cf, fch, fw)) bad = true;
+    double Fx[E], Fy[E], Fz[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) { Fx[e] = 0.0; Fy[e] = 0.0; Fz[e] = 0.0; }
 #pragma unroll
-    for (int q = 2; q <= NPH; ++q) {
+    for (int v = 0; v < V; ++v) {
+      s_fint += 0.5 * grad_term(P, v, gx[v], gy[v], gz[v]);
 #pragma unroll
-      for (int v = 0; v < V; ++v)
-        if (P->phase_of[v] == q) {
-          const double* g = P->gc + v * 9;
-          const double term = g[0] * (gx[v] * gx[v]) + g[4] * (gy[v] * gy[v]) + g[8] * (gz[v] * gz[v]) +
-                              (g[3] + g[1]) * gx[v] * gy[v] + (g[6] + g[2]) * gx[v] * gz[v] +
-                              (g[7] + g[5]) * gy[v] * gz[v];
-          s_fint += 0.5 * term;
-          const double fi = f[v];
-          double termch = Gf[q] - Gf[1];
-#pragma unroll
-          for (int e = 0; e < E; ++e) {
-            const double xd = b[NPH * e] - b[NPH * e + q - 1];
-            termch = termch + Mu[e] * xd;
-            const double tprc = Ppr[v] * xd;
-            Fx[e] += tprc * gx[v];
-            Fy[e] += tprc * gy[v];
-            Fz[e] += tprc * gz[v];
-          }
-          double wg = 2.0 * P->well[q - 1] * fi * (1.0 - fi) * (1.0 - 2.0 * fi);
-#pragma unroll
-          for (int jv = 0; jv < V; ++jv)
-            if (jv != v) wg = wg + 4.0 * P->well[0] * fi * (f[jv] * f[jv]);
-          fg[v] = termch * Ppr[v] + wg;
-        }
-      fch = fch + Pph[q] * Gf[q];
-      fw = fw + P->well[q - 1] * Gph[q];
+      for (int e = 0; e < E; ++e) {
+        Fx[e] += cf[v][e] * gx[v];
+        Fy[e] += cf[v][e] * gy[v];
+        Fz[e] += cf[v][e] * gz[v];
+      }
     }
-#undef GP
     s_fch += fch;
     s_fw += fw;
     // f_diff correction (:824-841)
@@ -358,16 +397,7 @@ k_chem(const DevParams* __restrict__ P, const T* __restrict__ phi, const T* __re
       }
     }
 #pragma unroll
-    for (int jv = 0; jv < V; ++jv) {
-      double t11 = 0.0;
-      int nt = 1;
-      if (pos[jv]) {
-#pragma unroll
-        for (int v = 0; v < V; ++v)
-          if (v != jv && pos[v]) { t11 = t11 + fg[v] - kd[v]; nt += 1; }
-      }
-      st(fdiff, jv * n3 + pt, fg[jv] - t11 / (double)nt);
-    }
+    for (int jv = 0; jv < V; ++jv) st(fdiff, jv * n3 + pt, fdiff_corr<V>(jv, fg, kd, pos));
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       st(flux, (3 * e + 0) * n3 + pt, Fx[e]);
@@ -797,6 +827,185 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
   }
   const double rsum = block_reduce<0, NW>(s_el, sh);
   if (w.t == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = rsum;
+}
+
+// x r2c of one component held as v[q] = (x[2n], x[2n+1]), n = tl + TPL q,
+// scaled by `scale`, into the S-layout half-spectrum out (rows past ny are
+// transformed but not stored).  Barriers inside: every thread of the
+// workgroup calls it.
+template <typename T, class P>
+__device__ __forceinline__ void x_r2c_store(const XRows<T, P>& w, C2<T> (&v)[P::E], C2<T>* lds, const C2<T>* twl,
+                                            const C2<T>* twn, C2<T>* __restrict__ out, int ny, T scale) {
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
+  const IdxLineMajor idx{w.bl * LP};
+  const T hs = (T)0.5 * scale;
+  __syncthreads();
+  line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, w.tl, lds, idx, twl);
+  if constexpr (P::NS > 1) line_sync<wave_lines<P>()>();
+#pragma unroll
+  for (int q = 0; q < E; ++q) lds[idx(w.tl + TPL * q)] = v[q];
+  __syncthreads();
+  if (w.yg < ny) {
+#pragma unroll
+    for (int i = 0; i <= E; ++i) {
+      const int kx = w.r + TPL * i;
+      if (kx > L) break;
+      const int k1 = kx == L ? 0 : kx, k2 = kx == 0 ? 0 : L - kx;
+      C2<T> zk = lds[w.bg * LP + k1 + (k1 >> 4)];
+      C2<T> zc = lds[w.bg * LP + k2 + (k2 >> 4)];
+      C2<T> ev = mk<T>(zk.x + zc.x, zk.y - zc.y);
+      C2<T> od = mk<T>(zk.y + zc.y, -(zk.x - zc.x));
+      C2<T> tw = twn[kx];
+      C2<T> xx = cadd<T>(ev, cmul<T>(tw, od));
+      out[((size_t)w.z * (L + 1) + kx) * ny + w.yg] = cscale<T>(xx, hs);
+    }
+  }
+}
+
+// ------------------------------------------ fused x pass of the chemistry
+// Multi_MEUMAPPS.f:575-845 after the gradient block's z and inverse y passes,
+// whose 4V x half-spectra (grad phi_v x, y, z and del.G del phi_v at
+// sgrad + (4v + d) nc) this kernel takes over: one workgroup owns B rows
+// (z, y0 .. y0+B-1).  It transforms del.G del phi of every variant to real
+// rows in registers, evaluates the KKS chemistry at every point of its rows
+// (phi and c read once), writes f_diff with the phi > 0 correction, then
+// transforms the gradient of one variant at a time and accumulates the
+// fluxes F_e = sum_v cf[v][e] grad phi_v and the gradient energy, and last
+// runs the x r2c transforms of c and F into sch + (4e + m) nc (m = 0: c,
+// 1-3: F_x, F_y, F_z), the input of the Cahn-Hilliard update's y pass.  The
+// gradients and fluxes never reach HBM as real fields.  Partial sums (f_int,
+// fch, fw) go to part[slot * nblk + b], b = blockIdx.y * gridDim.x + blockIdx.x.
+template <typename T, class P, int V, int NPH, int E>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
+k_x_chem(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const T* __restrict__ cons,
+         const C2<T>* __restrict__ sgrad, size_t nc, size_t n3, const C2<T>* __restrict__ twL,
+         const C2<T>* __restrict__ twN, int ny, T scale, T* __restrict__ fdiff, C2<T>* __restrict__ sch,
+         double* __restrict__ part, int* __restrict__ err, const int* __restrict__ gate) {
+  constexpr int L = P::L, TPL = P::TPL, EP = P::E, NT = TileLM<T, P>::THREADS, NW = (NT + 63) / 64;
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  __shared__ C2<T> twl[L], twn[2 * L];
+  __shared__ double sh[NW];
+  // the flux coefficients of this thread's points, kept in LDS between the
+  // chemistry and the gradient passes (slot ((q*2 + h)*V + v)*E + e, thread
+  // fastest): registers hold the transforms
+  __shared__ double cfl[EP * 2 * V * E * NT];
+  if (gated_off(gate)) return;
+  const XRows<T, P> w(ny);
+  for (int e = w.t; e < L; e += NT) twl[e] = twL[e];
+  for (int e = w.t; e < 2 * L; e += NT) twn[e] = twN[e];
+  __syncthreads();
+  const bool live = w.yl < ny;
+  // rows past ny read row ny-1 and store nothing
+  const size_t row = ((size_t)w.z * ny + (live ? w.yl : ny - 1)) * (size_t)(2 * L);
+  auto cfs = [&](int q, int h, int v, int e) -> double& { return cfl[(((q * 2 + h) * V + v) * E + e) * NT + w.t]; };
+  double s_fint = 0.0, s_fch = 0.0, s_fw = 0.0;
+  bool bad = false;
+  {
+    // 1. del.G del phi of every variant into registers
+    C2<T> kd[V][EP];
+    x_c2r_n<T, P, V>(w, sgrad + 3 * nc, 4 * nc, ny, lds, twl, twn, kd);
+    // 2. chemistry and f_diff at the 2 EP points of this thread
+#pragma unroll
+    for (int q = 0; q < EP; ++q) {
+      const int n = w.tl + TPL * q;
+      C2<T> ph[V], cr[E], fo[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) ph[v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[n];
+#pragma unroll
+      for (int e = 0; e < E; ++e) cr[e] = ((const C2<T>*)(cons + (size_t)e * n3 + row))[n];
+      // the two points of a pair one after the other (not unrolled: the
+      // chemistry's temporaries of both would be live at once)
+#pragma unroll 1
+      for (int h = 0; h < 2; ++h) {
+        double f[V], c[E], fg[V], kdv[V], cf[V][E], fch, fw;
+        bool pos[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          f[v] = (double)(h ? ph[v].y : ph[v].x);
+          kdv[v] = (double)(h ? kd[v][q].y : kd[v][q].x);
+          pos[v] = f[v] > 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) c[e] = Pp->cbar[e] + (double)(h ? cr[e].y : cr[e].x);
+        const bool ok = chem_point<V, NPH, E>(Pp, f, c, fg, cf, fch, fw);
+        if (live) {
+          bad = bad || !ok;
+          s_fch += fch;
+          s_fw += fw;
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) cfs(q, h, v, e) = cf[v][e];
+          const T d = (T)fdiff_corr<V>(v, fg, kdv, pos);
+          if (h) fo[v].y = d; else fo[v].x = d;
+        }
+      }
+      if (live) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) ((C2<T>*)(fdiff + (size_t)v * n3 + row))[n] = fo[v];
+      }
+      // one point pair at a time: keeps the loads of the next pair from
+      // being hoisted over this one's chemistry (register pressure)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // 3. gradients of one variant at a time: fluxes and gradient energy
+  double F[EP][2][3 * E];
+#pragma unroll
+  for (int q = 0; q < EP; ++q)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int m = 0; m < 3 * E; ++m) F[q][h][m] = 0.0;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    C2<T> g[3][EP];
+    __syncthreads();
+    x_c2r_n<T, P, 3>(w, sgrad + (size_t)(4 * v) * nc, nc, ny, lds, twl, twn, g);
+#pragma unroll
+    for (int q = 0; q < EP; ++q)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double gx = (double)(h ? g[0][q].y : g[0][q].x);
+        const double gy = (double)(h ? g[1][q].y : g[1][q].x);
+        const double gz = (double)(h ? g[2][q].y : g[2][q].x);
+        if (live) s_fint += 0.5 * grad_term(Pp, v, gx, gy, gz);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double cf = cfs(q, h, v, e);
+          F[q][h][3 * e + 0] += cf * gx;
+          F[q][h][3 * e + 1] += cf * gy;
+          F[q][h][3 * e + 2] += cf * gz;
+        }
+      }
+  }
+  // 4. x r2c of c and the fluxes (the values the unfused path would store)
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    C2<T> cr[EP];
+#pragma unroll
+    for (int q = 0; q < EP; ++q) cr[q] = ((const C2<T>*)(cons + (size_t)e * n3 + row))[w.tl + TPL * q];
+    x_r2c_store<T, P>(w, cr, lds, twl, twn, sch + (size_t)(4 * e) * nc, ny, scale);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      C2<T> v[EP];
+#pragma unroll
+      for (int q = 0; q < EP; ++q) v[q] = mk<T>((T)F[q][0][3 * e + d], (T)F[q][1][3 * e + d]);
+      x_r2c_store<T, P>(w, v, lds, twl, twn, sch + (size_t)(4 * e + 1 + d) * nc, ny, scale);
+    }
+  }
+  // 5. partial sums
+  const size_t nb = (size_t)gridDim.x * gridDim.y, b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const double r0 = block_reduce<0, NW>(s_fint, sh);
+  const double r1 = block_reduce<0, NW>(s_fch, sh);
+  const double r2 = block_reduce<0, NW>(s_fw, sh);
+  if (w.t == 0) {
+    part[b] = r0;
+    part[nb + b] = r1;
+    part[2 * nb + b] = r2;
+  }
+  if (bad) *err = MM_ERR_ZSOLVE;
 }
 
 // ------------------------------- Allen-Cahn inverse x pass fused with the clamp
@@ -1235,6 +1444,9 @@ template <typename T> struct Kern {
   // Allen-Cahn inverse x pass + clamp + check-loop sums: nblk gets the workgroup count
   bool (*xclamp)(bool launch, hipStream_t, const DevParams*, const C2<T>*, size_t, T*, const T*, size_t, const void*,
                  const void*, int, int, int, double*, const int*, int*) = nullptr;
+  // fused x pass of the chemistry (gradient c2r, KKS, f_diff, flux r2c): nblk gets the workgroup count
+  bool (*xchem)(bool launch, hipStream_t, const DevParams*, const T*, const T*, const C2<T>*, size_t, size_t,
+                const void*, const void*, int, int, int, T, T*, C2<T>*, double*, int*, const int*, int*) = nullptr;
 };
 
 // plan of the fused x kernel: six components of E points stay in registers
@@ -1292,6 +1504,29 @@ static bool go_xclamp(bool launch, hipStream_t s, const DevParams* P, const C2<T
     if (launch)
       hipLaunchKernelGGL((k_x_clamp<T, XP, V, E>), grid, dim3(TL::THREADS), 0, s, P, in, nc, phi, cons, n3,
                          (const C2<T>*)twL, (const C2<T>*)twN, ny, part, gate);
+    return true;
+  }
+}
+
+// the chemistry pass keeps the flux coefficients, fluxes and three gradient
+// components of 2E points per thread in registers: 4 complex points per
+// thread in both precisions
+template <int L> constexpr bool xchem_ok() { return PlanLean4<L>::E <= 4; }
+
+template <typename T, int L, int V, int NPH, int E>
+static bool go_xchem(bool launch, hipStream_t s, const DevParams* P, const T* phi, const T* cons,
+                     const C2<T>* sgrad, size_t nc, size_t n3, const void* twL, const void* twN, int ny, int nz,
+                     T scale, T* fdiff, C2<T>* sch, double* part, int* err, const int* gate, int* nblk) {
+  if constexpr (!xchem_ok<L>()) {
+    return false;
+  } else {
+    using XP = PlanLean4<L>;
+    using TL = TileLM<T, XP>;
+    dim3 grid((ny + TL::B - 1) / TL::B, nz);
+    if (nblk) *nblk = (int)(grid.x * grid.y);
+    if (launch)
+      hipLaunchKernelGGL((k_x_chem<T, XP, V, NPH, E>), grid, dim3(TL::THREADS), 0, s, P, phi, cons, sgrad, nc, n3,
+                         (const C2<T>*)twL, (const C2<T>*)twN, ny, scale, fdiff, sch, part, err, gate);
     return true;
   }
 }
@@ -1357,12 +1592,26 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
       default: return false;
     }
   }
+  static bool xchem(bool launch, hipStream_t s, const DevParams* P, const T* phi, const T* cons, const C2<T>* sgrad,
+                    size_t nc, size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, T scale, T* fdiff,
+                    C2<T>* sch, double* part, int* err, const int* gate, int* nblk) {
+    switch (nx / 2) {
+#define X(L_)                                                                                                     \
+  case L_:                                                                                                        \
+    return go_xchem<T, L_, V, NPH, E>(launch, s, P, phi, cons, sgrad, nc, n3, twL, twN, ny, nz, scale, fdiff, sch, \
+                                      part, err, gate, nblk);
+      X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+#undef X
+      default: return false;
+    }
+  }
   static Kern<T> table(bool homog) {
     Kern<T> k;
     k.chem = chem; k.ppoly = ppoly; k.clamp = clamp; k.zeroth = zeroth;
     // the fused x kernels keep V fields per point in registers: up to 4
     if constexpr (V <= 4) {
       k.xclamp = xclamp;
+      if constexpr (E == 1) k.xchem = xchem;
       k.xpolar = homog ? xpolar : nullptr;
       k.xelener = homog ? xelener : nullptr;
     }
@@ -1456,6 +1705,15 @@ struct mm_ctx {
   void* noise_pin = nullptr;
   int last_niter = 0;
   bool qchecked = false;  // the acoustic tensor passed k_qcheck
+  // fused chemistry pass (k_x_chem): the gradient block's inverse y pass
+  // writes its 4V spectra to sgrad (S layout), k_x_chem forms f_diff and the
+  // Cahn-Hilliard x spectra, and the Cahn-Hilliard update runs at the end of
+  // the head (ch_done); grad phi and del.G del phi are formed on demand only
+  bool xchem = false;
+  bool ch_done = false;
+  bool step_ran = false;  // a step has formed phi_hat since the last upload
+  bool rec_ch_head = false;  // the recorded graph's head runs the Cahn-Hilliard update
+  void* sgrad = nullptr;
   // branch tape (mm_branch_tape): caller's device buffer of br_cap steps,
   // br_stride bytes each; bt_now is the tape of the step being run (mode 0
   // outside run_step_eager)
@@ -1776,6 +2034,73 @@ template <typename T> int elastic_loop_eager(mm_ctx* c, Ctl& h, bool spectral) {
   return h.err;
 }
 
+template <typename T> int ch_update(mm_ctx* c, hipStream_t s, C2<T>* work, C2<T>* stg, int nst,
+                                    bool staged = false);
+
+// the gradient block's spectra (all V variants at once) into sgrad: z pass
+// with the gradient multipliers, then the inverse y pass to the S layout
+template <typename T> int grad_spectra(mm_ctx* c) {
+  const int V = c->V;
+  const size_t nc = c->nc;
+  C2<T>* work = (C2<T>*)c->swork;
+  OpGrad<T> op{c->dp, (C2<T>*)c->sphi, work, nc, 0};
+  op.gate = c->gate;
+  MM_TRY(launch_zfused<T>(c->fft, op, V, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 6 * V, nullptr));
+  MM_HIP(hipGetLastError());
+  void* zp[kMaxBatch];
+  void* sp[kMaxBatch];
+  for (int j = 0; j < 4 * V; ++j) { zp[j] = work + (size_t)j * nc; sp[j] = (C2<T>*)c->sgrad + (size_t)j * nc; }
+  return c->fft.y_pass_t<T>(zp, sp, 4 * V, +1, 2, c->s);
+}
+
+// head with the fused chemistry pass (phi_hat already formed): gradient
+// spectra, k_x_chem (f_diff and the Cahn-Hilliard x spectra in the stage),
+// then the rest of the Cahn-Hilliard update (:910-949), which needs nothing
+// from the elastic solve
+template <typename T> int head_xchem(mm_ctx* c) {
+  const size_t n3 = c->n3, nc = c->nc;
+  const Kern<T>& K = kern<T>(c);
+  MM_TRY(grad_spectra<T>(c));
+  C2<T>* stg = (C2<T>*)c->sstage;
+  int nblk = 0;
+  c->probe.begin(MM_PROBE_CHEM, c->s);
+  K.xchem(true, c->s, c->dp, (const T*)c->phi, (const T*)c->cons, (const C2<T>*)c->sgrad, nc, n3, c->fft.twLx,
+          c->fft.twNx, c->fft.nx, c->fft.ny, c->fft.nz, (T)(1.0 / (double)(float)n3), (T*)c->fdiff, stg, c->part,
+          c->derr, c->gate, &nblk);
+  c->probe.end(MM_PROBE_CHEM, c->s,
+               (double)sizeof(T) * ((double)(c->V + c->E + c->V) * n3 + 2.0 * (4 * c->V + 4 * c->E) * nc));
+  MM_HIP(hipGetLastError());
+  MM_TRY(finalize(c, nblk, 3, {0, 0, 0}, R_FINT));
+  MM_TRY(ch_update<T>(c, c->s, (C2<T>*)c->swork, stg, 4 * c->E, true));
+  c->ch_done = true;
+  c->step_ran = true;
+  return 0;
+}
+
+// grad phi and del.G del phi of the last step for mm_field (5, 6) on the
+// fused chemistry path, where the step does not form them: recomputed from
+// phi_hat, which the head's gradient z pass left as the full transform in
+// the column-tiled layout (back to S, k-space multipliers, full inverses)
+template <typename T> int grad_fields(mm_ctx* c) {
+  const int V = c->V;
+  const size_t nc = c->nc, n3 = c->n3;
+  C2<T>* sph = (C2<T>*)c->sgrad;
+  C2<T>* work = (C2<T>*)c->swork;
+  MM_TRY(c->fft.perm_sz<T>(c->sphi, sph, V, 2, c->s));
+  hipLaunchKernelGGL(k_ks_grad<T>, dim3(c->nblk_k, V), dim3(kBlock), 0, c->s, c->dp, (const C2<T>*)sph, work, 0, nc);
+  MM_HIP(hipGetLastError());
+  for (int v = 0; v < V; ++v) {
+    void* sp[4];
+    T* rp[4];
+    for (int d = 0; d < 4; ++d) sp[d] = work + (size_t)(4 * v + d) * nc;
+    for (int d = 0; d < 3; ++d) rp[d] = (T*)c->gphi + (size_t)(3 * v + d) * n3;
+    rp[3] = (T*)c->kdel + (size_t)v * n3;
+    MM_TRY(c->fft.inverse<T>(sp, rp, 4, c->s));
+  }
+  MM_HIP(hipStreamSynchronize(c->s));
+  return 0;
+}
+
 // -------------------------------------------------------------- one step
 // head: phi_hat, gradient block, chemistry (Multi_MEUMAPPS.f:575-845)
 template <typename T> int step_head(mm_ctx* c) {
@@ -1797,6 +2122,7 @@ template <typename T> int step_head(mm_ctx* c) {
   } else {
     MM_TRY(fwd<T>(c, phi, n3, sphi, V));
   }
+  if (c->xchem && !c->bt_now.mode) return head_xchem<T>(c);
   // 2. gradient block (:581-613) in groups of up to 4 variants
   const int G = std::max(1, std::min({V, c->nwork / 4, 4}));
   for (int v0 = 0; v0 < V; v0 += G) {
@@ -1849,11 +2175,18 @@ int finalize_clamp(mm_ctx* c, int nblk) {
 // Cahn-Hilliard update (Multi_MEUMAPPS.f:910-949) on stream s: work[4e] = c_hat,
 // work[4e+1..3] = F_hat (4E spectra), the fused passes staging through stg
 // (nst spectra; null = the context's stage); writes c
-template <typename T> int ch_update(mm_ctx* c, hipStream_t s, C2<T>* work, C2<T>* stg, int nst) {
+template <typename T> int ch_update(mm_ctx* c, hipStream_t s, C2<T>* work, C2<T>* stg, int nst, bool staged) {
   const size_t n3 = c->n3, nc = c->nc;
   const int E = c->E;
   const T scale = (T)(1.0 / (double)(float)n3);
-  for (int e = 0; e < E; ++e) {
+  if (staged) {
+    // the x spectra of c and the fluxes are in the stage (k_x_chem): y pass only
+    void* sp[kStageN];
+    void* op[kStageN];
+    for (int j = 0; j < 4 * E; ++j) { sp[j] = stg + (size_t)j * nc; op[j] = work + (size_t)j * nc; }
+    MM_TRY(c->fft.y_pass_t<T>(sp, op, 4 * E, -1, 1, s));
+  }
+  for (int e = 0; e < E && !staged; ++e) {
     const T* ip[4] = {(T*)c->cons + (size_t)e * n3, (T*)c->flux + (size_t)(3 * e) * n3,
                       (T*)c->flux + (size_t)(3 * e + 1) * n3, (T*)c->flux + (size_t)(3 * e + 2) * n3};
     void* op[4] = {work + (size_t)(4 * e) * nc, work + (size_t)(4 * e + 1) * nc, work + (size_t)(4 * e + 2) * nc,
@@ -1916,7 +2249,10 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
     MM_HIP(hipGetLastError());
   }
   // Cahn-Hilliard (:910-949), unless it already ran beside the elastic solve
-  if (c->ch_launched) {
+  // or at the end of the head (fused chemistry pass)
+  if (c->ch_done) {
+    c->ch_done = false;
+  } else if (c->ch_launched) {
     MM_HIP(hipStreamWaitEvent(c->s, c->ev_ch, 0));
     c->ch_launched = false;
   } else {
@@ -2054,7 +2390,7 @@ template <typename T> int run_step_eager_body(mm_ctx* c, mm_diag* d) {
   // it runs beside the elastic solve (joined before the Allen-Cahn update).
   // While a probe times a kernel class the update stays on the main stream,
   // so that the probed kernels run alone.
-  if (c->ch_async && c->probe.kind < 0) {
+  if (c->ch_async && c->probe.kind < 0 && !c->ch_done) {
     C2<T>* w = (C2<T>*)c->sch;
     MM_HIP(hipEventRecord(c->ev_head, c->s));
     MM_HIP(hipStreamWaitEvent(c->s2, c->ev_head, 0));
@@ -2082,6 +2418,7 @@ template <typename T> int run_step_eager_body(mm_ctx* c, mm_diag* d) {
 template <typename T> int record_step(mm_ctx* c, double* stage) {
   set_gate(c, nullptr);
   MM_TRY(step_head<T>(c));
+  c->rec_ch_head = c->ch_done;  // the recorded head runs the Cahn-Hilliard update
   hipLaunchKernelGGL(k_ctl_start, dim3(1), dim3(1), 0, c->s, c->ctl);
   MM_HIP(hipGetLastError());
   bool spectral = c->xpolar;
@@ -2111,6 +2448,7 @@ template <typename T> int finish_replay(mm_ctx* c, const double* st, mm_diag* d)
   std::memcpy(&h, st, sizeof(Ctl));
   if (!h.done) {
     set_gate(c, nullptr);
+    c->ch_done = c->rec_ch_head;  // the replayed head already updated c
     MM_TRY(elastic_loop_eager<T>(c, h, c->xpolar));
     MM_TRY(step_tail<T>(c, true));
     return step_collect(c, d);
@@ -2173,7 +2511,7 @@ void free_all(mm_ctx* c) {
   c->g = nullptr;
   void* bufs[] = {c->phi, c->cons, c->gphi, c->kdel, c->fdiff, c->flux, c->strain, c->xpol, c->el_en,
                   c->elgrad, c->sphi, c->suhat, c->swork, c->part, c->dres, c->derr, c->dp, c->ctl,
-                  c->dstage, c->sstage, c->sstrain, c->dflips};
+                  c->dstage, c->sstage, c->sstrain, c->dflips, c->sgrad};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->hres) (void)hipHostFree(c->hres);
@@ -2389,6 +2727,21 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
                     c->k32.xclamp(false, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, p->nx,
                                   p->ny, p->nz, nullptr, nullptr, &nb);
       if (nb > kMaxPartBlocks || c->V > kStageN) c->xclamp = false;
+      // the chemistry pass: MM_XCHEM=0 keeps the separate gradient x pass,
+      // chemistry kernel and flux transforms
+      const char* xc = std::getenv("MM_XCHEM");
+      if (!(xc && xc[0] == '0')) {
+        nb = 0;
+        if (precision == MM_FP64)
+          c->xchem = c->k64.xchem &&
+                     c->k64.xchem(false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx,
+                                  p->ny, p->nz, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr, &nb);
+        else
+          c->xchem = c->k32.xchem &&
+                     c->k32.xchem(false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx,
+                                  p->ny, p->nz, 0.0f, nullptr, nullptr, nullptr, nullptr, nullptr, &nb);
+        if (nb > kMaxPartBlocks || 4 * c->E > kStageN || 4 * c->V > kMaxBatch) c->xchem = false;
+      }
     }
     const char* zv = std::getenv("MM_ZF_ELASTIC_VARIANT");
     if (zv) c->zf_el_var = std::max(0, std::min(kZfElasticVariants - 1, std::atoi(zv)));
@@ -2416,6 +2769,7 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   rc = rc ? rc : dalloc(&c->elgrad, V * n3 * rb);
   rc = rc ? rc : dalloc(&c->sphi, V * nc * cb);
   rc = rc ? rc : dalloc(&c->suhat, 3 * nc * cb);
+  if (c->xchem) rc = rc ? rc : dalloc(&c->sgrad, (size_t)4 * V * nc * cb);
   rc = rc ? rc : dalloc(&c->swork, c->nwork * nc * cb);
   if (c->zfused) rc = rc ? rc : dalloc(&c->sstage, (size_t)kStageN * nc * cb);
   if (c->xpolar) rc = rc ? rc : dalloc(&c->sstrain, (size_t)6 * nc * cb);
@@ -2468,6 +2822,7 @@ extern "C" void mm_destroy(mm_ctx* c) {
 
 extern "C" int mm_upload(mm_ctx* c, const double* phi, const double* cons, const double* el_en, const double* u,
                          const double* gradu) {
+  if (c && phi) c->step_ran = false;  // phi_hat is stale
   if (!c) return MM_ERR_ARG;
   return c->prec == MM_FP64 ? upload<double>(c, phi, cons, el_en, u, gradu) : upload<float>(c, phi, cons, el_en, u, gradu);
 }
@@ -2570,6 +2925,11 @@ extern "C" int mm_field(mm_ctx* c, int32_t which, void** dev, int32_t* esz, int3
     case 6: p = c->kdel; n = c->V; break;
     case 7: p = c->elgrad; n = c->V; break;
     default: return MM_ERR_ARG;
+  }
+  if ((which == 5 || which == 6) && c->xchem && c->step_ran) {
+    // the fused chemistry pass does not form them: recompute from phi_hat
+    set_gate(c, nullptr);
+    MM_TRY(c->prec == MM_FP64 ? grad_fields<double>(c) : grad_fields<float>(c));
   }
   *dev = p;
   if (esz) *esz = (int32_t)c->esz;

@@ -499,6 +499,14 @@ int oracle_set_threads(oracle_t *o, int nthreads) {
   return 0;
 }
 
+/* the check-loop values of the last step (glob_min, glob_max, phi_max and the
+ * unnormalised solid sum), which the next fort.98 row reports */
+void oracle_get_lagged(oracle_t *o, double *v) {
+  v[0] = o->glob_min; v[1] = o->glob_max; v[2] = o->phi_max; v[3] = o->solid_frac;
+}
+void oracle_set_lagged(oracle_t *o, const double *v) {
+  o->glob_min = v[0]; o->glob_max = v[1]; o->phi_max = v[2]; o->solid_frac = v[3];
+}
 int oracle_get_step(oracle_t *o) { return o->step; }
 void oracle_set_step(oracle_t *o, int step) { o->step = step; }
 

@@ -36,6 +36,10 @@ int oracle_get_state(oracle_t *o, double *phi, double *cons, double *u, double *
  * 5 Fconx|Fcony|Fconz (3E), 6 P_phi (V) */
 int oracle_get_work(oracle_t *o, int which, double *out);
 int oracle_get_step(oracle_t *o);
+/* check-loop values of the last step (glob_min, glob_max, phi_max, solid
+ * sum): the lagged columns of the next row; get / set for checkpoints */
+void oracle_get_lagged(oracle_t *o, double *v4);
+void oracle_set_lagged(oracle_t *o, const double *v4);
 void oracle_set_step(oracle_t *o, int step);
 
 /* one full step of the loop body; returns 0 or an MM_ERR_* code */

@@ -75,6 +75,8 @@ def load():
         "oracle_get_state": (c_int, [c_void_p, D, D, D, D, D]),
         "oracle_get_work": (c_int, [c_void_p, c_int, D]),
         "oracle_get_step": (c_int, [c_void_p]),
+        "oracle_get_lagged": (None, [c_void_p, D]),
+        "oracle_set_lagged": (None, [c_void_p, D]),
         "oracle_set_step": (None, [c_void_p, c_int]),
         "oracle_step": (c_int, [c_void_p, POINTER(MMDiag)]),
         "oracle_f_trans": (c_int, [c_void_p, D, D]),
@@ -150,6 +152,15 @@ class Oracle:
     @property
     def step_count(self):
         return self.lib.oracle_get_step(self.h)
+
+    def get_lagged(self):
+        v = np.empty(4)
+        self.lib.oracle_get_lagged(self.h, _p(v))
+        return v
+
+    def set_lagged(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self.lib.oracle_set_lagged(self.h, _p(v))
 
     def set_step(self, s):
         self.lib.oracle_set_step(self.h, s)

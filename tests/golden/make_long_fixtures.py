@@ -8,6 +8,8 @@ squares, the minimum and the maximum; per step it keeps the fort.98 / fort.99
 row.  The file is rewritten after every dump, so an interrupted run leaves
 the dumps it reached.  The GPU test (tests/test_long_horizon_gpu.py) rebuilds
 the same initial fields from the same ran_2 seed and compares at each dump.
+With --ckpt the run keeps a full-state checkpoint every --ckpt-every steps
+and a rerun resumes from it.
 
     python tests/golden/make_long_fixtures.py workload1_64 [--threads 8]
     python tests/golden/make_long_fixtures.py grow_256
@@ -64,8 +66,9 @@ def main():
     ap.add_argument("name")
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--ckpt", default=None,
-                    help="directory for full-state checkpoints at every dump; a rerun resumes from "
-                         "the last one (its first row then has the lagged check-loop columns as NaN)")
+                    help="directory for full-state checkpoints (state, rows and dumps so far); a rerun "
+                         "resumes from the last one")
+    ap.add_argument("--ckpt-every", type=int, default=20)
     args = ap.parse_args()
     from oracle import Oracle
 
@@ -74,53 +77,49 @@ def main():
     phi, cons = initial_fields(w)
     o = Oracle(p, args.threads)
     o.set_state(phi=phi, cons=cons)
-    rows, dumps = [], []
-    start = every
+    diag, nel, cav = [], [], []
+    dumps = {k: [] for k in ("dump_steps", "phi_sub", "cons_sub", "phi_stats", "cons_stats")}
     ck = os.path.join(args.ckpt, args.name) if args.ckpt else None
-    if ck and os.path.exists(os.path.join(ck, "state.npz")) and os.path.exists(path(args.name)):
-        prev = dict(np.load(path(args.name)))
-        st = np.load(os.path.join(ck, "state.npz"))
-        if int(st["nsteps_done"]) == int(prev["nsteps_done"]):
-            o.set_state(phi=st["phi"], cons=st["cons"], u=st["u"], gradu=st["gradu"], el_en=st["el_en"])
-            o.set_step(int(st["step"]))
-            rows = [dict(step=0, n_elastic=int(ne), caviter=list(cv), **dict(zip(DIAG_KEYS, dg)))
-                    for dg, ne, cv in zip(prev["diag"], prev["n_elastic"], prev["caviter"])]
-            for i in range(len(prev["dump_steps"])):
-                dumps.append(dict(step=int(prev["dump_steps"][i]), phi_sub=prev["phi_sub"][i],
-                                  cons_sub=prev["cons_sub"][i], phi_stats=prev["phi_stats"][i],
-                                  cons_stats=prev["cons_stats"][i]))
-            start = len(rows) + every
-            print(f"{args.name}: resumed after {len(rows)} steps", flush=True)
+    ckf = os.path.join(ck, "run.npz") if ck else None
+    if ckf and os.path.exists(ckf):
+        st = np.load(ckf)
+        o.set_state(phi=st["phi"], cons=st["cons"], u=st["u"], gradu=st["gradu"], el_en=st["el_en"])
+        o.set_step(int(st["step"]))
+        o.set_lagged(st["lagged"])
+        diag, nel, cav = list(st["diag"]), list(st["n_elastic"]), list(st["caviter"])
+        for k in dumps:
+            if st[k].size:
+                dumps[k] = list(st[k])
+        print(f"{args.name}: resumed after {len(diag)} steps", flush=True)
     t0 = time.time()
-    for d in range(start, nsteps + 1, every):
-        new = o.step(every)
-        if rows and rows[-1].get("step") == 0:  # first row after a resume: lagged columns unknown
-            for k in ("glob_min", "glob_max", "phi_max", "solid_tot"):
-                new[0][k] = float("nan")
-        rows += new
+    while len(diag) < nsteps:
+        k = min(args.ckpt_every, every - len(diag) % every, nsteps - len(diag))
+        for r in o.step(k):
+            diag.append([r[q] for q in DIAG_KEYS])
+            nel.append(r["n_elastic"])
+            cav.append(r["caviter"][:p.numel])
         st = o.get_state()
-        if ck:
+        if len(diag) % every == 0:
+            dumps["dump_steps"].append(o.step_count - 1)
+            dumps["phi_sub"].append(st["phi"][:, ::stride, ::stride, ::stride].copy())
+            dumps["cons_sub"].append(st["cons"][:, ::stride, ::stride, ::stride].copy())
+            dumps["phi_stats"].append(field_stats(st["phi"]))
+            dumps["cons_stats"].append(field_stats(st["cons"]))
+            out = dict(name=args.name, seed=w.seed, amp=w.amp, stride=stride, every=every,
+                       nsteps_done=len(diag), diag=np.array(diag), n_elastic=np.array(nel, dtype=np.int32),
+                       caviter=np.array(cav), **{q: np.array(v) for q, v in dumps.items()})
+            tmp = path(args.name) + ".tmp.npz"
+            np.savez_compressed(tmp, **out)
+            os.replace(tmp, path(args.name))
+            print(f"{args.name}: step {o.step_count - 1} ({len(diag)} steps, {time.time() - t0:.0f} s) "
+                  f"phi_max {diag[-1][7]:.4e} solid {diag[-1][8]:.4e}", flush=True)
+        if ckf:
             os.makedirs(ck, exist_ok=True)
-            np.savez(os.path.join(ck, "state.tmp.npz"), nsteps_done=d, step=o.step_count, **st)
-            os.replace(os.path.join(ck, "state.tmp.npz"), os.path.join(ck, "state.npz"))
-        dumps.append(dict(step=rows[-1]["step"], phi_sub=st["phi"][:, ::stride, ::stride, ::stride].copy(),
-                          cons_sub=st["cons"][:, ::stride, ::stride, ::stride].copy(),
-                          phi_stats=field_stats(st["phi"]), cons_stats=field_stats(st["cons"])))
-        out = dict(name=args.name, seed=w.seed, amp=w.amp, stride=stride, every=every,
-                   nsteps_done=len(rows),
-                   diag=np.array([[r[k] for k in DIAG_KEYS] for r in rows]),
-                   n_elastic=np.array([r["n_elastic"] for r in rows], dtype=np.int32),
-                   caviter=np.array([r["caviter"][:p.numel] for r in rows]),
-                   dump_steps=np.array([x["step"] for x in dumps], dtype=np.int32),
-                   phi_sub=np.stack([x["phi_sub"] for x in dumps]),
-                   cons_sub=np.stack([x["cons_sub"] for x in dumps]),
-                   phi_stats=np.stack([x["phi_stats"] for x in dumps]),
-                   cons_stats=np.stack([x["cons_stats"] for x in dumps]))
-        tmp = path(args.name) + ".tmp.npz"
-        np.savez_compressed(tmp, **out)
-        os.replace(tmp, path(args.name))
-        print(f"{args.name}: step {rows[-1]['step']} ({len(rows)} steps, {time.time() - t0:.0f} s) "
-              f"phi_max {rows[-1]['phi_max']:.4e} solid {rows[-1]['solid_tot']:.4e}", flush=True)
+            tmp = os.path.join(ck, "run.tmp.npz")
+            np.savez(tmp, step=o.step_count, lagged=o.get_lagged(), diag=np.array(diag),
+                     n_elastic=np.array(nel, dtype=np.int32), caviter=np.array(cav),
+                     **{q: np.array(v) for q, v in dumps.items()}, **st)
+            os.replace(tmp, ckf)
 
 
 if __name__ == "__main__":
