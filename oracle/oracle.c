@@ -21,6 +21,10 @@
  *   clamp + check loop .............. Multi_MEUMAPPS.f:1008-1065
  *   Langevin noise .................. Multi_MEUMAPPS.f:552-571, 963-978 with
  *                                    ran_2.f:1-43 and langevin ModFunc.f:77-100
+ *   tangent-linear step ............. forward-mode derivative of the above
+ *                                    (oracle_tlm_*), the reference gradient
+ *                                    of the adjoint checks; no reference
+ *                                    counterpart
  *
  * Deviations that only move bits in the last place: the FFT is this file's
  * own mixed-radix Cooley-Tukey (not FFTW); the LINPACK rcond estimate is
@@ -348,6 +352,7 @@ struct oracle {
   double noise_sd;
   int32_t r_idum, r_state[35];
   double *phi_noise; /* V*N3, accumulated over the noise steps */
+  struct tlm *t;     /* tangent-linear state (oracle_tlm_set), or NULL */
 };
 
 #define RIDX(o, x, y, z) ((long)(x) + (long)(o)->nx * ((long)(y) + (long)(o)->ny * (long)(z)))
@@ -409,6 +414,7 @@ void oracle_destroy(oracle_t *o) {
                     o->fdiff, o->elgrad, o->Pphi, o->Fx, o->Fy, o->Fz, o->xf, o->u_old, o->qinv};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) free(bufs[i]);
   free(o->sphi); free(o->sw); free(o->phi_noise);
+  oracle_tlm_off(o);
   free(o);
 }
 
@@ -943,6 +949,461 @@ static int elener(oracle_t *o) {
   return err ? MM_ERR_CINV_ENER : 0;
 }
 
+/* ------------------------------------------------- tangent-linear step --
+ * Forward-mode derivative of oracle_step along a direction of the state
+ * (phi, c, grad u) and of the parameter arrays grad_coeff, wellph, cmobs,
+ * b_mob and trans_ppt (mm_adj_grads layout).  The tangent runs in lockstep
+ * with the primal and takes the primal's decisions: the phi > 0 tests of the
+ * f_diff correction (Multi_MEUMAPPS.f:824-841), the clamp (:1034-1035), the
+ * elastic iteration count and its cfac sequence (ModElastic.f:433-488).  So
+ * it is the derivative of the map with those decisions frozen.  Each block is
+ * the differential of the primal block above it, written from the primal's
+ * formulas; the compliance mixtures are differentiated through their inverses
+ * (dC = -C dM C), so inhomogeneous moduli are covered too.  Regular steps
+ * only (no noise, step != elastep0). */
+struct tlm {
+  double *dphi, *dcons, *dgu;                        /* tangent state       */
+  double *dgx, *dgy, *dgz, *dkd, *dfd, *dPp, *dFx, *dFy, *dFz, *dxf;
+  cplx *dsphi, *sw;                                  /* V + 12 spectra      */
+  mm_adj_grads dp;                                   /* parameter direction */
+};
+
+/* sum dG_ij k_i k_j with the direction of grad_coeff (the 2-form of kgk) */
+static inline double dkgk(const oracle_t *o, int v, int kx, int ky, int kz) {
+  const double *G = o->t->dp.grad_coeff + v * 9; /* G[j*3+i] = (i,j) */
+  double k1q = o->kqx[kx], k2q = o->kqy[ky], k3q = o->kqz[kz];
+  double a1 = o->kfx[kx], a2 = o->kfy[ky], a3 = o->kfz[kz];
+  return G[0] * k1q + G[4] * k2q + G[8] * k3q + (G[3] + G[1]) * (-(a1 * a2)) +
+         (G[7] + G[5]) * (-(a2 * a3)) + (G[2] + G[6]) * (-(a3 * a1));
+}
+
+/* d(num / den) = dnum / den - num dden / den^2 */
+static inline cplx dquot(cplx num, cplx dnum, double den, double dden) {
+  return dnum / den - num * (dden / (den * den));
+}
+
+/* gradient block: dg = F^-1(i k dphi^), dkd = F^-1(kgk dphi^ + dkgk phi^) */
+static void tlm_gradients(oracle_t *o) {
+  struct tlm *t = o->t;
+  long n3 = o->n3, nc = o->nc;
+  int H = o->H, ny = o->ny, nt = o->nthreads;
+  for (int v = 0; v < o->V; ++v) {
+    cplx *dS = t->dsphi + v * nc;
+    const cplx *S = o->sphi + v * nc;
+    f_trans(o, t->dphi + v * n3, dS);
+    cplx *gx = t->sw, *gy = t->sw + nc, *gz = t->sw + 2 * nc, *kd = t->sw + 3 * nc;
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long i = 0; i < nc; ++i) {
+      int kx = (int)(i % H), ky = (int)((i / H) % ny), kz = (int)(i / ((long)H * ny));
+      gx[i] = mul_i(dS[i], o->kfx[kx]);
+      gy[i] = mul_i(dS[i], o->kfy[ky]);
+      gz[i] = mul_i(dS[i], o->kfz[kz]);
+      kd[i] = dS[i] * kgk(o, v, kx, ky, kz) + S[i] * dkgk(o, v, kx, ky, kz);
+    }
+    inv_trans(o, gx, t->dgx + v * n3);
+    inv_trans(o, gy, t->dgy + v * n3);
+    inv_trans(o, gz, t->dgz + v * n3);
+    inv_trans(o, kd, t->dkd + v * n3);
+  }
+}
+
+/* chemistry at one point: dP_phi, the flux tangents and d f_diff (before the
+ * elastic term).  The primal quantities are recomputed here from the
+ * unchanged phi, c, grad phi and kdel. */
+static void tlm_chem_point(oracle_t *o, long pt) {
+  struct tlm *t = o->t;
+  const int V = o->V, E = o->E, NPH = o->NPH, zdim = o->zdim;
+  long n3 = o->n3;
+  double phi[MM_MAX_VAR], dph[MM_MAX_VAR], Ppr[MM_MAX_VAR], dPpr[MM_MAX_VAR];
+  double Pph[MM_MAX_PH + 1] = {0}, dPph[MM_MAX_PH + 1] = {0};
+  for (int v = 0; v < V; ++v) {
+    double fi = o->phi[v * n3 + pt], df = t->dphi[v * n3 + pt], fi_sq = fi * fi;
+    phi[v] = fi;
+    dph[v] = df;
+    double pl = fi_sq * fi * (6.0 * fi_sq - 15.0 * fi + 10.0);
+    Ppr[v] = 30.0 * fi_sq * (fi_sq - 2.0 * fi + 1.0);
+    dPpr[v] = 60.0 * fi * (1.0 - fi) * (1.0 - 2.0 * fi) * df;
+    double dpl = Ppr[v] * df;
+    t->dPp[v * n3 + pt] = dpl;
+    int ph = o->phase_of[v];
+    Pph[1] += pl; Pph[ph] += pl;
+    dPph[1] += dpl; dPph[ph] += dpl;
+  }
+  Pph[1] = 1.0 - Pph[1];
+  dPph[1] = -dPph[1];
+  /* Zintf and its right-hand side as in chem_point; solve Z s = r */
+  double Z[16 * 16], Zr[16];
+  memset(Z, 0, sizeof(Z));
+  for (int e = 1; e <= E; ++e) {
+    int i1 = e + E + 1;
+    for (int ph = 1; ph <= NPH - 1; ++ph) {
+      int i0 = (NPH - 1) * (e - 1) + ph, j0 = NPH * (e - 1) + ph;
+      Z[(i0 - 1) + (j0 - 1) * zdim] = GP(o, e, ph);
+      Z[(i0 - 1) + j0 * zdim] = -GP(o, e, ph + 1);
+      Zr[i0 - 1] = GP(o, e, ph) * GP(o, i1, ph) - GP(o, e, ph + 1) * GP(o, i1, ph + 1);
+    }
+  }
+  for (int e = 1; e <= E; ++e) {
+    int i0 = zdim - E + e;
+    Zr[i0 - 1] = o->cons[(e - 1) * n3 + pt];
+    for (int ph = 1; ph <= NPH; ++ph) Z[(i0 - 1) + (NPH * (e - 1) + ph - 1) * zdim] = Pph[ph];
+  }
+  if (oracle_invert(Z, zdim, MM_ERR_ZSOLVE)) return; /* the primal reports it */
+  double sol[16], rhs[16] = {0}, dsol[16];
+  for (int i = 0; i < zdim; ++i) {
+    double a = 0.0;
+    for (int j = 0; j < zdim; ++j) a += Z[i + j * zdim] * Zr[j];
+    sol[i] = a;
+  }
+  /* d s = Z^-1 (dr - dZ s): dZ lives in the mixture rows, dr in c */
+  for (int e = 1; e <= E; ++e) {
+    int i0 = zdim - E + e;
+    double r = t->dcons[(e - 1) * n3 + pt];
+    for (int ph = 1; ph <= NPH; ++ph) r -= dPph[ph] * sol[NPH * (e - 1) + ph - 1];
+    rhs[i0 - 1] = r;
+  }
+  for (int i = 0; i < zdim; ++i) {
+    double a = 0.0;
+    for (int j = 0; j < zdim; ++j) a += Z[i + j * zdim] * rhs[j];
+    dsol[i] = a;
+  }
+  double X[MM_MAX_EL + 1][MM_MAX_PH + 1], dX[MM_MAX_EL + 1][MM_MAX_PH + 1];
+  for (int e = 1; e <= E; ++e)
+    for (int ph = 1; ph <= NPH; ++ph) {
+      X[e][ph] = sol[NPH * (e - 1) + ph - 1];
+      dX[e][ph] = dsol[NPH * (e - 1) + ph - 1];
+    }
+  double Gf[MM_MAX_PH + 1], dGf[MM_MAX_PH + 1], Mu[MM_MAX_EL + 1], dMu[MM_MAX_EL + 1];
+  for (int ph = 1; ph <= NPH; ++ph) {
+    double g = 0.0, dg = 0.0;
+    for (int e = 1; e <= E; ++e) {
+      double d = X[e][ph] - GP(o, e + E + 1, ph);
+      g += GP(o, e, ph) * d * d;
+      dg += 2.0 * GP(o, e, ph) * d * dX[e][ph];
+    }
+    Gf[ph] = g + GP(o, E + 1, ph);
+    dGf[ph] = dg;
+  }
+  for (int e = 1; e <= E; ++e) {
+    Mu[e] = 2.0 * GP(o, e, 1) * (X[e][1] - GP(o, e + E + 1, 1));
+    dMu[e] = 2.0 * GP(o, e, 1) * dX[e][1];
+  }
+  const double *w = o->p.wellph, *dw = t->dp.wellph;
+  double dF[3][MM_MAX_EL + 1], dfg[MM_MAX_VAR];
+  memset(dF, 0, sizeof(dF));
+  for (int v = 0; v < V; ++v) {
+    int ph = o->phase_of[v];
+    double g[3] = {o->gradx[v * n3 + pt], o->grady[v * n3 + pt], o->gradz[v * n3 + pt]};
+    double dg[3] = {t->dgx[v * n3 + pt], t->dgy[v * n3 + pt], t->dgz[v * n3 + pt]};
+    double fi = phi[v], df = dph[v];
+    double termch = Gf[ph] - Gf[1], dtermch = dGf[ph] - dGf[1];
+    for (int e = 1; e <= E; ++e) {
+      double xd = X[e][1] - X[e][ph], dxd = dX[e][1] - dX[e][ph];
+      termch += Mu[e] * xd;
+      dtermch += dMu[e] * xd + Mu[e] * dxd;
+      for (int d = 0; d < 3; ++d)
+        dF[d][e] += dPpr[v] * xd * g[d] + Ppr[v] * dxd * g[d] + Ppr[v] * xd * dg[d];
+    }
+    double h = fi * (1.0 - fi) * (1.0 - 2.0 * fi), dh = (1.0 - 6.0 * fi + 6.0 * fi * fi) * df;
+    double dwg = 2.0 * dw[ph - 1] * h + 2.0 * w[ph - 1] * dh;
+    for (int jv = 0; jv < V; ++jv)
+      if (jv != v) {
+        double pj = phi[jv], dpj = dph[jv];
+        dwg += 4.0 * (dw[0] * fi * pj * pj + w[0] * df * pj * pj + w[0] * fi * 2.0 * pj * dpj);
+      }
+    dfg[v] = dtermch * Ppr[v] + termch * dPpr[v] + dwg;
+  }
+  for (int e = 1; e <= E; ++e) {
+    t->dFx[(e - 1) * n3 + pt] = dF[0][e];
+    t->dFy[(e - 1) * n3 + pt] = dF[1][e];
+    t->dFz[(e - 1) * n3 + pt] = dF[2][e];
+  }
+  /* the correction with the primal's phi > 0 decisions */
+  for (int jv = 0; jv < V; ++jv) {
+    double dt11 = 0.0;
+    int nt = 1;
+    if (phi[jv] > 0.0)
+      for (int v = 0; v < V; ++v)
+        if (phi[v] > 0.0 && v != jv) {
+          dt11 += dfg[v] - t->dkd[v * n3 + pt];
+          nt += 1;
+        }
+    t->dfd[jv * n3 + pt] = dfg[jv] - dt11 / (double)nt;
+  }
+}
+
+static void tlm_chemistry(oracle_t *o) {
+#pragma omp parallel for num_threads(o->nthreads) schedule(static)
+  for (long pt = 0; pt < o->n3; ++pt) tlm_chem_point(o, pt);
+}
+
+/* the compliance mixture at one point, its inverse C and dC = -C dM C.
+ * guard: the (1 - sum P) matrix term only while sum P < 1 (eldis), or always
+ * (elener). */
+static int tlm_moduli(const oracle_t *o, long pt, int guard, double *C, double *dC) {
+  const struct tlm *t = o->t;
+  long n3 = o->n3;
+  double M[36] = {0}, dM[36] = {0}, ps = 0.0, dps = 0.0;
+  for (int v = 0; v < o->V; ++v) {
+    double pl = o->Pphi[v * n3 + pt], dpl = t->dPp[v * n3 + pt];
+    const double *S = o->p.sela + (o->phase_of[v] - 1) * 36;
+    ps += pl;
+    dps += dpl;
+    for (int q = 0; q < 36; ++q) { M[q] += pl * S[q]; dM[q] += dpl * S[q]; }
+  }
+  if (!guard || ps < 1.0)
+    for (int q = 0; q < 36; ++q) { M[q] += (1.0 - ps) * o->p.sela[q]; dM[q] -= dps * o->p.sela[q]; }
+  memcpy(C, M, sizeof(M));
+  if (oracle_invert(C, 6, MM_ERR_CINV_ITER)) return 1;
+  double T[36];
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) {
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += dM[a + 6 * c] * C[c + 6 * b];
+      T[a + 6 * b] = s;
+    }
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) {
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += C[a + 6 * c] * T[c + 6 * b];
+      dC[a + 6 * b] = -s;
+    }
+  return 0;
+}
+
+/* one elastic iteration: the polarization stress tangent from the
+ * pre-iteration grad u, then its k-space map to the new d grad u */
+static void tlm_elastic_iter(oracle_t *o, double cfac) {
+  struct tlm *t = o->t;
+  const int V = o->V, H = o->H, ny = o->ny, nt = o->nthreads;
+  long n3 = o->n3, nc = o->nc;
+  const int vmap[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (long pt = 0; pt < n3; ++pt) {
+    double C[36], dC[36];
+    if (tlm_moduli(o, pt, 1, C, dC)) continue;
+    double a[6] = {0}, da[6] = {0}; /* sum_v P_v eps0_v and its tangent */
+    for (int kl = 0; kl < 6; ++kl)
+      for (int v = 0; v < V; ++v) {
+        a[kl] += o->Pphi[v * n3 + pt] * TP(o, kl, v);
+        da[kl] += t->dPp[v * n3 + pt] * TP(o, kl, v) + o->Pphi[v * n3 + pt] * t->dp.trans_ppt[v * 6 + kl];
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = i; j < 3; ++j) {
+        int ij = vmap[i][j];
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            int kl = vmap[k][l];
+            double b = o->gu[(l * 3 + k) * n3 + pt] + o->p.e_mean[kl];
+            double db = t->dgu[(l * 3 + k) * n3 + pt];
+            double Cp = o->p.cela[36 + ij + 6 * kl] - C[ij + 6 * kl];
+            s += cfac * (-dC[ij + 6 * kl] * b + Cp * db) + dC[ij + 6 * kl] * a[kl] + C[ij + 6 * kl] * da[kl];
+          }
+        t->dxf[ij * n3 + pt] = s;
+      }
+  }
+  cplx *sxx = (cplx *)malloc(sizeof(cplx) * 6 * nc);
+  for (int c = 0; c < 6; ++c) f_trans(o, t->dxf + c * n3, sxx + c * nc);
+  cplx *sg = t->sw; /* 9 spectra of d grad u */
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (long idx = 0; idx < nc; ++idx) {
+    int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+    double kf[3] = {o->kfx[kx], o->kfy[ky], o->kfz[kz]};
+    cplx uh[3] = {0, 0, 0};
+    if (kx || ky || kz) {
+      cplx R[3] = {0, 0, 0};
+      for (int i = 0; i < 3; ++i)
+        for (int d = 0; d < 3; ++d) R[i] += mul_i(sxx[vmap[i][d] * nc + idx], kf[d]);
+      const double *Q = o->qinv + 9 * idx;
+      for (int i = 0; i < 3; ++i) uh[i] = Q[i] * R[0] + Q[i + 3] * R[1] + Q[i + 6] * R[2];
+    }
+    for (int d = 0; d < 3; ++d)
+      for (int i = 0; i < 3; ++i) sg[(d * 3 + i) * nc + idx] = mul_i(uh[i], kf[d]);
+  }
+  free(sxx);
+  for (int c = 0; c < 9; ++c) inv_trans(o, sg + c * nc, t->dgu + c * n3);
+}
+
+/* elener: d el_grad added to d f_diff (el_en is not carried) */
+static void tlm_elener(oracle_t *o) {
+  struct tlm *t = o->t;
+  const int V = o->V;
+  long n3 = o->n3;
+  const double thr = (double)1.e-9f;
+  const double wv[6] = {1, 1, 1, 2, 2, 2};
+#pragma omp parallel for num_threads(o->nthreads) schedule(static)
+  for (long pt = 0; pt < n3; ++pt) {
+    double C[36], dC[36];
+    if (tlm_moduli(o, pt, 0, C, dC)) continue;
+    double te[6] = {0}, dte[6] = {0};
+    for (int v = 0; v < V; ++v)
+      for (int m = 0; m < 6; ++m) {
+        te[m] += o->Pphi[v * n3 + pt] * TP(o, m, v);
+        dte[m] += t->dPp[v * n3 + pt] * TP(o, m, v) + o->Pphi[v * n3 + pt] * t->dp.trans_ppt[v * 6 + m];
+      }
+    const double *g = o->gu, *dg = t->dgu;
+#define GU(A, i, j) A[((j) * 3 + (i)) * n3 + pt]
+    double eps[6] = {o->p.e_mean[0] + GU(g, 0, 0) - te[0], o->p.e_mean[1] + GU(g, 1, 1) - te[1],
+                     o->p.e_mean[2] + GU(g, 2, 2) - te[2],
+                     2.0 * o->p.e_mean[3] + GU(g, 2, 1) + GU(g, 1, 2) - 2.0 * te[3],
+                     2.0 * o->p.e_mean[4] + GU(g, 2, 0) + GU(g, 0, 2) - 2.0 * te[4],
+                     2.0 * o->p.e_mean[5] + GU(g, 0, 1) + GU(g, 1, 0) - 2.0 * te[5]};
+    double deps[6] = {GU(dg, 0, 0) - dte[0], GU(dg, 1, 1) - dte[1], GU(dg, 2, 2) - dte[2],
+                      GU(dg, 2, 1) + GU(dg, 1, 2) - 2.0 * dte[3], GU(dg, 2, 0) + GU(dg, 0, 2) - 2.0 * dte[4],
+                      GU(dg, 0, 1) + GU(dg, 1, 0) - 2.0 * dte[5]};
+#undef GU
+    double sig[6], dsig[6];
+    for (int a = 0; a < 6; ++a) {
+      double s = 0.0, ds = 0.0;
+      for (int m = 0; m < 6; ++m) {
+        s += C[a + 6 * m] * eps[m];
+        ds += dC[a + 6 * m] * eps[m] + C[a + 6 * m] * deps[m];
+      }
+      sig[a] = s;
+      dsig[a] = ds;
+    }
+    /* 0.5 sig.S sig and its tangent for the compliance of phase ph */
+#define QUAD(ph, q, dq)                                                                   \
+  do {                                                                                    \
+    double q_ = 0.0, dq_ = 0.0;                                                           \
+    for (int a = 0; a < 6; ++a)                                                           \
+      for (int m = 0; m < 6; ++m) {                                                       \
+        double s_ = SE(o, a, m, ph);                                                      \
+        q_ += sig[a] * s_ * sig[m];                                                       \
+        dq_ += dsig[a] * s_ * sig[m] + sig[a] * s_ * dsig[m];                             \
+      }                                                                                   \
+    q = 0.5 * q_;                                                                         \
+    dq = 0.5 * dq_;                                                                       \
+  } while (0)
+    double el_mat, del_mat;
+    QUAD(1, el_mat, del_mat);
+    for (int v = 0; v < V; ++v) {
+      double ts = 0.0;
+      for (int m = 0; m < 6; ++m) ts += TP(o, m, v);
+      if (!(ts > thr)) continue;
+      double el_ppt, del_ppt;
+      QUAD(o->phase_of[v], el_ppt, del_ppt);
+      double eg = 0.0, deg = 0.0;
+      for (int m = 0; m < 6; ++m) {
+        eg += wv[m] * sig[m] * TP(o, m, v);
+        deg += wv[m] * (dsig[m] * TP(o, m, v) + sig[m] * t->dp.trans_ppt[v * 6 + m]);
+      }
+      double fi = o->phi[v * n3 + pt], df = t->dphi[v * n3 + pt];
+      double pp = 30.0 * fi * fi * (fi * fi - 2.0 * fi + 1.0);
+      double dpp = 60.0 * fi * (1.0 - fi) * (1.0 - 2.0 * fi) * df;
+      t->dfd[v * n3 + pt] += dpp * (el_mat - el_ppt - eg) + pp * (del_mat - del_ppt - deg);
+    }
+#undef QUAD
+  }
+}
+
+/* Cahn-Hilliard: c^' = (c^ + dtm i k.F^) / (1 - dtm k^2), dtm = dt M_e */
+static void tlm_ch(oracle_t *o) {
+  struct tlm *t = o->t;
+  const int H = o->H, ny = o->ny, nt = o->nthreads;
+  long n3 = o->n3, nc = o->nc;
+  cplx *s = t->sw; /* c, Fx, Fy, Fz, then their tangents */
+  for (int e = 0; e < o->E; ++e) {
+    f_trans(o, o->cons + e * n3, s);
+    f_trans(o, o->Fx + e * n3, s + nc);
+    f_trans(o, o->Fy + e * n3, s + 2 * nc);
+    f_trans(o, o->Fz + e * n3, s + 3 * nc);
+    f_trans(o, t->dcons + e * n3, s + 4 * nc);
+    f_trans(o, t->dFx + e * n3, s + 5 * nc);
+    f_trans(o, t->dFy + e * n3, s + 6 * nc);
+    f_trans(o, t->dFz + e * n3, s + 7 * nc);
+    double dtm = o->p.t_step * o->p.cmobs[e], ddtm = o->p.t_step * t->dp.cmobs[e];
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long idx = 0; idx < nc; ++idx) {
+      int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+      double ksq = o->kqx[kx] + o->kqy[ky] + o->kqz[kz];
+      cplx t1 = mul_i(s[nc + idx], o->kfx[kx]) + mul_i(s[2 * nc + idx], o->kfy[ky]) +
+                mul_i(s[3 * nc + idx], o->kfz[kz]);
+      cplx dt1 = mul_i(s[5 * nc + idx], o->kfx[kx]) + mul_i(s[6 * nc + idx], o->kfy[ky]) +
+                 mul_i(s[7 * nc + idx], o->kfz[kz]);
+      cplx num = s[idx] + dtm * t1, dnum = s[4 * nc + idx] + ddtm * t1 + dtm * dt1;
+      s[4 * nc + idx] = dquot(num, dnum, 1.0 - dtm * ksq, -ddtm * ksq);
+    }
+    inv_trans(o, s + 4 * nc, t->dcons + e * n3);
+  }
+}
+
+/* Allen-Cahn: phi^' = (phi^ - dtl fd^) / (1 - dtl kgk), dtl = dt L; the
+ * tangent of the pre-clamp field */
+static void tlm_ac(oracle_t *o) {
+  struct tlm *t = o->t;
+  const int H = o->H, ny = o->ny, nt = o->nthreads;
+  long n3 = o->n3, nc = o->nc;
+  cplx *sf = t->sw, *dsf = t->sw + nc, *out = t->sw + 2 * nc;
+  double dtl = o->p.t_step * o->p.b_mob, ddtl = o->p.t_step * t->dp.b_mob;
+  for (int v = 0; v < o->V; ++v) {
+    const cplx *S = o->sphi + v * nc, *dS = t->dsphi + v * nc;
+    f_trans(o, o->fdiff + v * n3, sf);
+    f_trans(o, t->dfd + v * n3, dsf);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long idx = 0; idx < nc; ++idx) {
+      int kx = (int)(idx % H), ky = (int)((idx / H) % ny), kz = (int)(idx / ((long)H * ny));
+      double kg = kgk(o, v, kx, ky, kz), dkg = dkgk(o, v, kx, ky, kz);
+      cplx num = S[idx] - dtl * sf[idx], dnum = dS[idx] - ddtl * sf[idx] - dtl * dsf[idx];
+      out[idx] = dquot(num, dnum, 1.0 - dtl * kg, -(ddtl * kg + dtl * dkg));
+    }
+    inv_trans(o, out, t->dphi + v * n3);
+  }
+}
+
+/* clamp: the tangent passes where 0 <= phi' <= 1 (primal pre-clamp field) */
+static void tlm_clamp(oracle_t *o) {
+  long n = (long)o->V * o->n3;
+  for (long i = 0; i < n; ++i)
+    if (o->phi[i] < 0.0 || o->phi[i] > 1.0) o->t->dphi[i] = 0.0;
+}
+
+void oracle_tlm_off(oracle_t *o) {
+  struct tlm *t = o->t;
+  if (!t) return;
+  double *b[] = {t->dphi, t->dcons, t->dgu, t->dgx, t->dgy, t->dgz, t->dkd, t->dfd,
+                 t->dPp, t->dFx, t->dFy, t->dFz, t->dxf};
+  for (size_t i = 0; i < sizeof(b) / sizeof(b[0]); ++i) free(b[i]);
+  free(t->dsphi); free(t->sw);
+  free(t);
+  o->t = NULL;
+}
+
+int oracle_tlm_set(oracle_t *o, const double *dphi, const double *dcons, const double *dgradu,
+                   const mm_adj_grads *dpar) {
+  long n3 = o->n3, V = o->V, E = o->E;
+  if (!o->t) {
+    struct tlm *t = (struct tlm *)calloc(1, sizeof(struct tlm));
+    if (!t) return MM_ERR_NOMEM;
+    o->t = t;
+    t->dphi = xcalloc(V * n3, 8); t->dcons = xcalloc(E * n3, 8); t->dgu = xcalloc(9 * n3, 8);
+    t->dgx = xcalloc(V * n3, 8); t->dgy = xcalloc(V * n3, 8); t->dgz = xcalloc(V * n3, 8);
+    t->dkd = xcalloc(V * n3, 8); t->dfd = xcalloc(V * n3, 8); t->dPp = xcalloc(V * n3, 8);
+    t->dFx = xcalloc(E * n3, 8); t->dFy = xcalloc(E * n3, 8); t->dFz = xcalloc(E * n3, 8);
+    t->dxf = xcalloc(6 * n3, 8);
+    t->dsphi = xcalloc(V * o->nc, sizeof(cplx));
+    t->sw = xcalloc(12 * o->nc, sizeof(cplx));
+    if (!t->dphi || !t->dgu || !t->dxf || !t->dsphi || !t->sw) { oracle_tlm_off(o); return MM_ERR_NOMEM; }
+  }
+  struct tlm *t = o->t;
+  if (dphi) memcpy(t->dphi, dphi, 8 * V * n3); else memset(t->dphi, 0, 8 * V * n3);
+  if (dcons) memcpy(t->dcons, dcons, 8 * E * n3); else memset(t->dcons, 0, 8 * E * n3);
+  if (dgradu) memcpy(t->dgu, dgradu, 8 * 9 * n3); else memset(t->dgu, 0, 8 * 9 * n3);
+  if (dpar) t->dp = *dpar; else memset(&t->dp, 0, sizeof(t->dp));
+  return 0;
+}
+
+int oracle_tlm_get(oracle_t *o, double *dphi, double *dcons, double *dgradu) {
+  struct tlm *t = o->t;
+  long n3 = o->n3;
+  if (!t) return MM_ERR_ARG;
+  if (dphi) memcpy(dphi, t->dphi, 8 * o->V * n3);
+  if (dcons) memcpy(dcons, t->dcons, 8 * o->E * n3);
+  if (dgradu) memcpy(dgradu, t->dgu, 8 * 9 * n3);
+  return 0;
+}
+
 /* slab-ordered global sums of the six convergence terms, ModElastic.f:413-430 */
 static void err_sums(oracle_t *o, double err[6]) {
   int nz = o->nz;
@@ -1066,6 +1527,7 @@ int oracle_eldis(oracle_t *o, int *n_iter, double *cfac_out) {
         }
     }
     if (ierr) return MM_ERR_CINV_ITER;
+    if (o->t) tlm_elastic_iter(o, cfac);
     cplx *sx = o->sw; /* 6 spectra; then reused for u (3) and grad u (9) */
     cplx *sxx = (cplx *)malloc(sizeof(cplx) * 6 * nc);
     for (int c = 0; c < 6; ++c) f_trans(o, o->xf + c * n3, sxx + c * nc);
@@ -1160,6 +1622,7 @@ int oracle_step(oracle_t *o, mm_diag *d) {
   int rc;
   const int noisy = o->step <= o->p.noise_step;
   if (noisy && !o->noise_on) return MM_ERR_ARG;
+  if (o->t && (noisy || o->step == o->p.elastep0)) return MM_ERR_ARG;
   /* noise, :552-571: one langevin draw per point and variant, z/y/x/variant order */
   if (noisy) {
     for (int z = 0; z < nz; ++z)
@@ -1174,13 +1637,16 @@ int oracle_step(oracle_t *o, mm_diag *d) {
   memset(&dd, 0, sizeof(dd));
   dd.step = o->step;
   gradients(o);
+  if (o->t) tlm_gradients(o);
   double fint, fch, fw;
   if ((rc = chemistry(o, &fint, &fch, &fw))) return rc;
+  if (o->t) tlm_chemistry(o);
   memset(o->elgrad, 0, sizeof(double) * V * n3);
   int niter = 0;
   double cfac = 1.0;
   if (o->step >= o->p.elastep0) {
     if ((rc = oracle_eldis(o, &niter, &cfac))) return rc;
+    if (o->t) tlm_elener(o);
   } else {
     memset(o->el_en, 0, sizeof(double) * n3);
   }
@@ -1201,6 +1667,7 @@ int oracle_step(oracle_t *o, mm_diag *d) {
   dd.n_elastic = niter;
   dd.cfac = cfac;
   /* Cahn-Hilliard, :910-949 */
+  if (o->t) tlm_ch(o);
   cplx *sc = o->sw, *sfx = o->sw + nc, *sfy = o->sw + 2 * nc, *sfz = o->sw + 3 * nc;
   for (int e = 0; e < E; ++e) {
     f_trans(o, o->cons + e * n3, sc);
@@ -1220,6 +1687,7 @@ int oracle_step(oracle_t *o, mm_diag *d) {
     inv_trans(o, sc, o->cons + e * n3);
   }
   /* Allen-Cahn, :955-1001 (term3 = t_step F(phi_noise), zero past noise_step) */
+  if (o->t) tlm_ac(o);
   cplx *sf = o->sw + nc, *sn = o->sw + 2 * nc;
   for (int v = 0; v < V; ++v) {
     cplx *sp = o->sphi + v * nc; /* FFT of phi from the gradient block; phi unchanged since */
@@ -1238,6 +1706,7 @@ int oracle_step(oracle_t *o, mm_diag *d) {
     inv_trans(o, sp, o->phi + v * n3);
   }
   /* check loop, :1008-1065 */
+  if (o->t) tlm_clamp(o);
   {
     long slab = (long)o->nx * o->ny;
     double *pf = calloc((size_t)nz * (4 + E), sizeof(double));

@@ -82,6 +82,19 @@ int oracle_elener(oracle_t *o);
  * d*3+i); either output may be NULL */
 int oracle_elastic_zeroth(oracle_t *o, const double *pphi, double *u, double *gradu);
 
+/* Tangent-linear step (forward-mode derivative along a direction).  Once a
+ * direction is set, every oracle_step also advances the tangent of
+ * (phi, c, grad u), taking the primal's branch decisions (phi > 0 in the
+ * f_diff correction, the clamp, the elastic iteration count and cfac
+ * sequence).  dpar is a direction in the parameter arrays grad_coeff,
+ * wellph, cmobs, b_mob and trans_ppt (mm_adj_grads layout); NULL pointers
+ * are zero directions.  A noise step or step == elastep0 returns MM_ERR_ARG
+ * while a tangent is set. */
+int oracle_tlm_set(oracle_t *o, const double *dphi, const double *dcons, const double *dgradu,
+                   const mm_adj_grads *dpar);
+int oracle_tlm_get(oracle_t *o, double *dphi, double *dcons, double *dgradu);
+void oracle_tlm_off(oracle_t *o);
+
 #ifdef __cplusplus
 }
 #endif

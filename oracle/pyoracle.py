@@ -12,7 +12,7 @@ from ctypes import POINTER, c_double, c_int, c_void_p
 
 import numpy as np
 
-from meumapps_ss_amd.params import MMDiag, MMParams
+from meumapps_ss_amd.params import MMAdjGrads, MMDiag, MMParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # MM_ORACLE_LIB points the tests at another build of oracle.c (a copy with a
@@ -93,6 +93,9 @@ def load():
         "oracle_noise_accumulate": (None, [POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), c_double, c_int,
                                            ctypes.c_long, D]),
         "oracle_get_noise": (c_int, [c_void_p, POINTER(ctypes.c_int32), POINTER(ctypes.c_int32), D]),
+        "oracle_tlm_set": (c_int, [c_void_p, D, D, D, POINTER(MMAdjGrads)]),
+        "oracle_tlm_get": (c_int, [c_void_p, D, D, D]),
+        "oracle_tlm_off": (None, [c_void_p]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -220,6 +223,28 @@ class Oracle:
         (rc, el_en, el_grad)."""
         rc = self.lib.oracle_elener(self.h)
         return rc, self.get_state()["el_en"], self.get_work(4)
+
+    def tlm_set(self, dphi=None, dcons=None, dgradu=None, dpar: MMAdjGrads = None):
+        """Start the tangent-linear run: every following step also advances
+        the tangent of (phi, c, grad u) along this direction (state arrays
+        and a parameter direction in the mm_adj_grads layout; None = 0)."""
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        dphi, dcons, dgradu = map(conv, (dphi, dcons, dgradu))
+        rc = self.lib.oracle_tlm_set(self.h, _p(dphi), _p(dcons), _p(dgradu),
+                                     None if dpar is None else ctypes.byref(dpar))
+        if rc:
+            raise RuntimeError(f"oracle_tlm_set error {rc}")
+
+    def tlm_get(self):
+        V, E = self.V, self.E
+        dphi = np.empty((V, *self.shape)); dcons = np.empty((E, *self.shape)); dgu = np.empty((9, *self.shape))
+        rc = self.lib.oracle_tlm_get(self.h, _p(dphi), _p(dcons), _p(dgu))
+        if rc:
+            raise RuntimeError(f"oracle_tlm_get error {rc}")
+        return dict(phi=dphi, cons=dcons, gradu=dgu)
+
+    def tlm_off(self):
+        self.lib.oracle_tlm_off(self.h)
 
     def eldis(self):
         n = c_int(0)
