@@ -21,6 +21,10 @@ int launch_zfused(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int p
 template <typename T>
 int launch_zf_elastic(const Fft3& f, const OpElastic<T>& op, int v, hipStream_t s, int probe_kind, double bytes,
                       int* nblk);
+template <typename T>
+int launch_y_ch(const Fft3& f, const DevParams* P, const void* in, void* out, int ne, int e0, hipStream_t s);
+template <typename T>
+int launch_y_grad(const Fft3& f, const DevParams* P, const void* in, void* out, int nv, hipStream_t s);
 
 // ============================================================== kernels
 
@@ -67,6 +71,12 @@ __global__ void __launch_bounds__(kBlock) k_fill(T* __restrict__ p, size_t n, T 
 // Allen-Cahn source with Langevin noise (Multi_MEUMAPPS.f:963-973):
 // F(phi) - dt b F(f_diff) + dt F(phi_noise) = F(phi) - dt b F(f_diff - phi_noise / b),
 // so the noise joins f_diff before the one forward transform
+// a[i] += b[i] (f_diff + el_grad formed on demand)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_add_to(T* __restrict__ a, const T* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    a[i] = (T)((double)a[i] + (double)b[i]);
+}
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_noise_fold(const T* __restrict__ fdiff, const T* __restrict__ noise,
                                                         T* __restrict__ out, size_t n, double b_mob) {
@@ -711,6 +721,39 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
   }
 }
 
+// x r2c of one component held as v[q] = (x[2n], x[2n+1]), n = tl + TPL q,
+// scaled by `scale`, into the S-layout half-spectrum out (rows past ny are
+// transformed but not stored).  Barriers inside: every thread of the
+// workgroup calls it.
+template <typename T, class P>
+__device__ __forceinline__ void x_r2c_store(const XRows<T, P>& w, C2<T> (&v)[P::E], C2<T>* lds, const C2<T>* twl,
+                                            const C2<T>* twn, C2<T>* __restrict__ out, int ny, T scale) {
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
+  const IdxLineMajor idx{w.bl * LP};
+  const T hs = (T)0.5 * scale;
+  __syncthreads();
+  line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, w.tl, lds, idx, twl);
+  if constexpr (P::NS > 1) line_sync<wave_lines<P>()>();
+#pragma unroll
+  for (int q = 0; q < E; ++q) lds[idx(w.tl + TPL * q)] = v[q];
+  __syncthreads();
+  if (w.yg < ny) {
+#pragma unroll
+    for (int i = 0; i <= E; ++i) {
+      const int kx = w.r + TPL * i;
+      if (kx > L) break;
+      const int k1 = kx == L ? 0 : kx, k2 = kx == 0 ? 0 : L - kx;
+      C2<T> zk = lds[w.bg * LP + k1 + (k1 >> 4)];
+      C2<T> zc = lds[w.bg * LP + k2 + (k2 >> 4)];
+      C2<T> ev = mk<T>(zk.x + zc.x, zk.y - zc.y);
+      C2<T> od = mk<T>(zk.y + zc.y, -(zk.x - zc.x));
+      C2<T> tw = twn[kx];
+      C2<T> xx = cadd<T>(ev, cmul<T>(tw, od));
+      out[((size_t)w.z * (L + 1) + kx) * ny + w.yg] = cscale<T>(xx, hs);
+    }
+  }
+}
+
 // Elastic energy density and driving force at one point when every phase has
 // the same moduli (ModElastic.f:493-691): sig = C eps with
 // eps = strain + w e_mean - w sum_v p_v eps0_v, every phase's energy density
@@ -771,11 +814,15 @@ __device__ __forceinline__ double elener_homog(const DevParams* __restrict__ P, 
 // evaluates elener (homogeneous moduli) at every point of its rows: el_en,
 // el_grad, f_diff += el_grad, and the partial sums of el_en per workgroup
 // (part[blockIdx.y * gridDim.x + blockIdx.x]).  The real strain is not written.
-template <typename T, class P, int V, int NPH>
+// XS: f_diff + el_grad is not stored as a real field but run through the x r2c
+// transform (scaled by `scale`) into the half-spectra sfd + v nc, the input of
+// the Allen-Cahn update's y pass; fdiff is then only read.
+template <typename T, class P, int V, int NPH, bool XS>
 __global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
 k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<T>* __restrict__ sstrain, size_t nc,
            size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T* __restrict__ el_en,
-           T* __restrict__ elgrad, T* __restrict__ fdiff, double* __restrict__ part, const int* __restrict__ gate) {
+           T* __restrict__ elgrad, T* __restrict__ fdiff, C2<T>* __restrict__ sfd, T scale,
+           double* __restrict__ part, const int* __restrict__ gate) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, NW = (TileLM<T, P>::THREADS + 63) / 64;
   __shared__ C2<T> lds[TileLM<T, P>::LDS];
   __shared__ C2<T> twl[L], twn[2 * L];
@@ -790,6 +837,11 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
   // rows past ny read row ny-1 and store nothing
   const size_t row = ((size_t)w.z * ny + (w.yl < ny ? w.yl : ny - 1)) * (size_t)(2 * L);
   double s_el = 0.0;
+  C2<T> fx[XS ? V : 1][E];  // XS: f_diff of this thread's points for the r2c
+#pragma unroll
+  for (int v = 0; v < (XS ? V : 1); ++v)
+#pragma unroll
+    for (int q = 0; q < E; ++q) fx[v][q] = mk<T>(0, 0);
   if (w.yl < ny) {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -821,45 +873,17 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         ((C2<T>*)(elgrad + (size_t)v * n3 + row))[n] = go[v];
-        ((C2<T>*)(fdiff + (size_t)v * n3 + row))[n] = fd[v];
+        if constexpr (XS) fx[v][q] = fd[v];
+        else ((C2<T>*)(fdiff + (size_t)v * n3 + row))[n] = fd[v];
       }
     }
   }
+  if constexpr (XS) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) x_r2c_store<T, P>(w, fx[v], lds, twl, twn, sfd + (size_t)v * nc, ny, scale);
+  }
   const double rsum = block_reduce<0, NW>(s_el, sh);
   if (w.t == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = rsum;
-}
-
-// x r2c of one component held as v[q] = (x[2n], x[2n+1]), n = tl + TPL q,
-// scaled by `scale`, into the S-layout half-spectrum out (rows past ny are
-// transformed but not stored).  Barriers inside: every thread of the
-// workgroup calls it.
-template <typename T, class P>
-__device__ __forceinline__ void x_r2c_store(const XRows<T, P>& w, C2<T> (&v)[P::E], C2<T>* lds, const C2<T>* twl,
-                                            const C2<T>* twn, C2<T>* __restrict__ out, int ny, T scale) {
-  constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
-  const IdxLineMajor idx{w.bl * LP};
-  const T hs = (T)0.5 * scale;
-  __syncthreads();
-  line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, w.tl, lds, idx, twl);
-  if constexpr (P::NS > 1) line_sync<wave_lines<P>()>();
-#pragma unroll
-  for (int q = 0; q < E; ++q) lds[idx(w.tl + TPL * q)] = v[q];
-  __syncthreads();
-  if (w.yg < ny) {
-#pragma unroll
-    for (int i = 0; i <= E; ++i) {
-      const int kx = w.r + TPL * i;
-      if (kx > L) break;
-      const int k1 = kx == L ? 0 : kx, k2 = kx == 0 ? 0 : L - kx;
-      C2<T> zk = lds[w.bg * LP + k1 + (k1 >> 4)];
-      C2<T> zc = lds[w.bg * LP + k2 + (k2 >> 4)];
-      C2<T> ev = mk<T>(zk.x + zc.x, zk.y - zc.y);
-      C2<T> od = mk<T>(zk.y + zc.y, -(zk.x - zc.x));
-      C2<T> tw = twn[kx];
-      C2<T> xx = cadd<T>(ev, cmul<T>(tw, od));
-      out[((size_t)w.z * (L + 1) + kx) * ny + w.yg] = cscale<T>(xx, hs);
-    }
-  }
 }
 
 // ------------------------------------------ fused x pass of the chemistry
@@ -1439,8 +1463,9 @@ template <typename T> struct Kern {
   bool (*xpolar)(bool launch, hipStream_t, const DevParams*, const T*, const C2<T>*, C2<T>*, size_t, size_t,
                  const void*, const void*, int, int, int, T, double, const double*, const int*) = nullptr;
   // fused x c2r of the strain + elener (homogeneous moduli): nblk gets the workgroup count
+  // (sfd non-null: f_diff goes out as x half-spectra into sfd, scaled by scale)
   bool (*xelener)(bool launch, hipStream_t, const DevParams*, const T*, const C2<T>*, size_t, size_t, const void*,
-                  const void*, int, int, int, T*, T*, T*, double*, const int*, int*) = nullptr;
+                  const void*, int, int, int, T*, T*, T*, C2<T>*, T, double*, const int*, int*) = nullptr;
   // Allen-Cahn inverse x pass + clamp + check-loop sums: nblk gets the workgroup count
   bool (*xclamp)(bool launch, hipStream_t, const DevParams*, const C2<T>*, size_t, T*, const T*, size_t, const void*,
                  const void*, int, int, int, double*, const int*, int*) = nullptr;
@@ -1475,7 +1500,7 @@ static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* p
 template <typename T, int L, int V, int NPH>
 static bool go_xelener(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* sstrain, size_t nc,
                        size_t n3, const void* twL, const void* twN, int ny, int nz, T* el_en, T* elgrad, T* fdiff,
-                       double* part, const int* gate, int* nblk) {
+                       C2<T>* sfd, T scale, double* part, const int* gate, int* nblk) {
   if constexpr (!xplan_ok<T, L>()) {
     return false;
   } else {
@@ -1483,9 +1508,16 @@ static bool go_xelener(bool launch, hipStream_t s, const DevParams* P, const T* 
     using TL = TileLM<T, XP>;
     dim3 grid((ny + TL::B - 1) / TL::B, nz);
     if (nblk) *nblk = (int)(grid.x * grid.y);
+    if constexpr (sizeof(T) == 8) {
+      if (launch && sfd) {
+        hipLaunchKernelGGL((k_x_elener<T, XP, V, NPH, true>), grid, dim3(TL::THREADS), 0, s, P, phi, sstrain, nc,
+                           n3, (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, sfd, scale, part, gate);
+        return true;
+      }
+    }
     if (launch)
-      hipLaunchKernelGGL((k_x_elener<T, XP, V, NPH>), grid, dim3(TL::THREADS), 0, s, P, phi, sstrain, nc, n3,
-                         (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, part, gate);
+      hipLaunchKernelGGL((k_x_elener<T, XP, V, NPH, false>), grid, dim3(TL::THREADS), 0, s, P, phi, sstrain, nc, n3,
+                         (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, sfd, scale, part, gate);
     return true;
   }
 }
@@ -1569,12 +1601,12 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
   }
   static bool xelener(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* sstrain, size_t nc,
                       size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, T* el_en, T* elgrad,
-                      T* fdiff, double* part, const int* gate, int* nblk) {
+                      T* fdiff, C2<T>* sfd, T scale, double* part, const int* gate, int* nblk) {
     switch (nx / 2) {
 #define X(L_)                                                                                                  \
   case L_:                                                                                                     \
     return go_xelener<T, L_, V, NPH>(launch, s, P, phi, sstrain, nc, n3, twL, twN, ny, nz, el_en, elgrad, fdiff, \
-                                     part, gate, nblk);
+                                     sfd, scale, part, gate, nblk);
       X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 #undef X
       default: return false;
@@ -1712,6 +1744,9 @@ struct mm_ctx {
   bool xchem = false;
   bool ch_done = false;
   bool step_ran = false;  // a step has formed phi_hat since the last upload
+  bool xs_tail = false;   // the last tail run or recorded handed f_diff over as x half-spectra
+  bool xs_graph = false;  // the same for the recorded step of graph mode
+  bool fd_split = false;  // fdiff holds the chemistry's f_diff without el_grad
   bool rec_ch_head = false;  // the recorded graph's head runs the Cahn-Hilliard update
   void* sgrad = nullptr;
   // branch tape (mm_branch_tape): caller's device buffer of br_cap steps,
@@ -2043,14 +2078,13 @@ template <typename T> int grad_spectra(mm_ctx* c) {
   const int V = c->V;
   const size_t nc = c->nc;
   C2<T>* work = (C2<T>*)c->swork;
-  OpGrad<T> op{c->dp, (C2<T>*)c->sphi, work, nc, 0};
+  // z pass: phi_hat, phi_hat i kf_z and phi_hat G_zz kq_z per variant; the
+  // inverse y pass applies the kx and ky factors and forms the 4V spectra
+  OpGrad3<T> op{c->dp, (C2<T>*)c->sphi, work, nc};
   op.gate = c->gate;
-  MM_TRY(launch_zfused<T>(c->fft, op, V, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 6 * V, nullptr));
+  MM_TRY(launch_zfused<T>(c->fft, op, V, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 5 * V, nullptr));
   MM_HIP(hipGetLastError());
-  void* zp[kMaxBatch];
-  void* sp[kMaxBatch];
-  for (int j = 0; j < 4 * V; ++j) { zp[j] = work + (size_t)j * nc; sp[j] = (C2<T>*)c->sgrad + (size_t)j * nc; }
-  return c->fft.y_pass_t<T>(zp, sp, 4 * V, +1, 2, c->s);
+  return launch_y_grad<T>(c->fft, c->dp, work, c->sgrad, V, c->s);
 }
 
 // head with the fused chemistry pass (phi_hat already formed): gradient
@@ -2179,35 +2213,42 @@ template <typename T> int ch_update(mm_ctx* c, hipStream_t s, C2<T>* work, C2<T>
   const size_t n3 = c->n3, nc = c->nc;
   const int E = c->E;
   const T scale = (T)(1.0 / (double)(float)n3);
-  if (staged) {
-    // the x spectra of c and the fluxes are in the stage (k_x_chem): y pass only
-    void* sp[kStageN];
-    void* op[kStageN];
-    for (int j = 0; j < 4 * E; ++j) { sp[j] = stg + (size_t)j * nc; op[j] = work + (size_t)j * nc; }
-    MM_TRY(c->fft.y_pass_t<T>(sp, op, 4 * E, -1, 1, s));
-  }
-  for (int e = 0; e < E && !staged; ++e) {
-    const T* ip[4] = {(T*)c->cons + (size_t)e * n3, (T*)c->flux + (size_t)(3 * e) * n3,
-                      (T*)c->flux + (size_t)(3 * e + 1) * n3, (T*)c->flux + (size_t)(3 * e + 2) * n3};
-    void* op[4] = {work + (size_t)(4 * e) * nc, work + (size_t)(4 * e + 1) * nc, work + (size_t)(4 * e + 2) * nc,
-                   work + (size_t)(4 * e + 3) * nc};
-    if (c->zfused) {
-      MM_TRY(fwd_xy<T>(c, ip, (C2<T>* const*)op, 4, s, stg, nst));
+  if (!stg) stg = (C2<T>*)c->sstage;
+  if (c->zfused) {
+    // x pass of c and the fluxes into the stage (already done by k_x_chem when
+    // staged), then the y pass with the x and y flux terms folded in: work[2e]
+    // = c_hat + dt M (i kf_x F_x + i kf_y F_y), work[2e+1] = F_z
+    if (staged) {
+      MM_TRY(launch_y_ch<T>(c->fft, c->dp, stg, work, E, 0, s));
     } else {
+      if (nst < 4) return MM_ERR_ARG;
+      for (int e = 0; e < E; ++e) {
+        const T* ip[4] = {(T*)c->cons + (size_t)e * n3, (T*)c->flux + (size_t)(3 * e) * n3,
+                          (T*)c->flux + (size_t)(3 * e + 1) * n3, (T*)c->flux + (size_t)(3 * e + 2) * n3};
+        void* sp4[4];
+        for (int m = 0; m < 4; ++m) sp4[m] = stg + (size_t)m * nc;
+        MM_TRY(c->fft.x_r2c<T>(ip, sp4, 4, s, scale));
+        MM_TRY(launch_y_ch<T>(c->fft, c->dp, stg, work + (size_t)(2 * e) * nc, 1, e, s));
+      }
+    }
+    OpCH2<T> opch{c->dp, work, nc};
+    opch.gate = c->gate;
+    MM_TRY(launch_zfused<T>(c->fft, opch, E, s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 3 * E, nullptr));
+  } else {
+    for (int e = 0; e < E; ++e) {
+      const T* ip[4] = {(T*)c->cons + (size_t)e * n3, (T*)c->flux + (size_t)(3 * e) * n3,
+                        (T*)c->flux + (size_t)(3 * e + 1) * n3, (T*)c->flux + (size_t)(3 * e + 2) * n3};
+      void* op[4] = {work + (size_t)(4 * e) * nc, work + (size_t)(4 * e + 1) * nc, work + (size_t)(4 * e + 2) * nc,
+                     work + (size_t)(4 * e + 3) * nc};
       MM_TRY(c->fft.forward<T>(ip, op, 4, s, scale));
     }
-  }
-  if (c->zfused) {
-    OpCH<T> opch{c->dp, work, nc};
-    opch.gate = c->gate;
-    MM_TRY(launch_zfused<T>(c->fft, opch, E, s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 5 * E, nullptr));
-  } else {
     hipLaunchKernelGGL(k_ks_ch<T>, dim3(c->nblk_k, E), dim3(kBlock), 0, s, c->dp, work, nc);
   }
   MM_HIP(hipGetLastError());
   C2<T>* sp[MM_MAX_EL];
   T* rp[MM_MAX_EL];
-  for (int e = 0; e < E; ++e) { sp[e] = work + (size_t)(4 * e) * nc; rp[e] = (T*)c->cons + (size_t)e * n3; }
+  const int cs = c->zfused ? 2 : 4;  // spacing of the new c_hat in work
+  for (int e = 0; e < E; ++e) { sp[e] = work + (size_t)(cs * e) * nc; rp[e] = (T*)c->cons + (size_t)e * n3; }
   if (c->zfused) {
     MM_TRY(inv_yx<T>(c, sp, rp, E, s, stg, nst));
   } else {
@@ -2227,13 +2268,25 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   C2<T>* sphi = (C2<T>*)c->sphi;
   C2<T>* work = (C2<T>*)c->swork;
   const dim3 gp(c->nblk_pw);
+  // With the fused x kernels and the Cahn-Hilliard update already done, elener
+  // hands f_diff to the Allen-Cahn update as x half-spectra in the stage (the
+  // real f_diff + el_grad is not formed; fdiff keeps the chemistry's part and
+  // mm_field adds el_grad on demand).  Noise steps and recorded steps need the
+  // real f_diff.  Binary64 only: the binary32 kernel (8 points per thread)
+  // spills with the three extra transforms and runs slower than the separate
+  // x pass (0.32 -> 0.55 ms at 256^3), the binary64 one saves 0.06 ms.
+  const bool xs = sizeof(T) == 8 && elastic && c->xpolar && c->zfused && c->ch_done &&
+                  c->step > c->prm.noise_step && !rec_fd;
+  c->xs_tail = xs;
+  c->fd_split = xs;
   if (elastic) {
     c->probe.begin(MM_PROBE_ELENER, c->s);
     int nblk = c->nblk_pw;
     if (c->xpolar) {
       // the strain's x c2r fused with elener (the real strain is not formed)
       K.xelener(true, c->s, c->dp, phi, (const C2<T>*)c->sstrain, nc, n3, c->fft.twLx, c->fft.twNx, c->fft.nx,
-                c->fft.ny, c->fft.nz, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->gate, &nblk);
+                c->fft.ny, c->fft.nz, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff,
+                xs ? (C2<T>*)c->sstage : nullptr, (T)(1.0 / (double)(float)n3), c->part, c->gate, &nblk);
     } else {
       K.elener(gp, c->s, c->dp, phi, (T*)c->strain, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->derr,
                n3, c->gate);
@@ -2274,7 +2327,14 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
       const T* fin[MM_MAX_VAR];
       C2<T>* fout[MM_MAX_VAR];
       for (int v = 0; v < V; ++v) { fin[v] = fsrc + (size_t)v * n3; fout[v] = work + (size_t)v * nc; }
-      MM_TRY(fwd_xy<T>(c, fin, fout, V));
+      if (xs) {
+        // the x half-spectra of f_diff are in the stage already
+        void* st[MM_MAX_VAR];
+        for (int v = 0; v < V; ++v) st[v] = (C2<T>*)c->sstage + (size_t)v * nc;
+        MM_TRY(c->fft.y_pass_t<T>(st, (void* const*)fout, V, -1, 1, c->s));
+      } else {
+        MM_TRY(fwd_xy<T>(c, fin, fout, V));
+      }
       OpAC<T> opac{c->dp, work, sphi, nc};
       opac.gate = c->gate;
       MM_TRY(launch_zfused<T>(c->fft, opac, V, c->s, MM_PROBE_ZF_OTHER, (double)nc * 2 * sizeof(T) * 3 * V, nullptr));
@@ -2433,6 +2493,7 @@ template <typename T> int record_step(mm_ctx* c, double* stage) {
   }
   set_gate(c, &c->ctl->done);
   MM_TRY(step_tail<T>(c, true));
+  c->xs_graph = c->xs_tail;
   set_gate(c, nullptr);
   hipLaunchKernelGGL(k_gather, dim3(1), dim3(64), 0, c->s, (const Ctl*)c->ctl, (const double*)c->dres,
                      (const int*)c->derr, stage);
@@ -2446,6 +2507,7 @@ template <typename T> int record_step(mm_ctx* c, double* stage) {
 template <typename T> int finish_replay(mm_ctx* c, const double* st, mm_diag* d) {
   Ctl h;
   std::memcpy(&h, st, sizeof(Ctl));
+  c->fd_split = c->xs_graph;  // the replayed tail's hand-over of f_diff
   if (!h.done) {
     set_gate(c, nullptr);
     c->ch_done = c->rec_ch_head;  // the replayed head already updated c
@@ -2486,6 +2548,7 @@ template <typename T> int run_step(mm_ctx* c, mm_diag* d) {
   MM_HIP(hipStreamSynchronize(c->s));
   return finish_replay<T>(c, c->hstage, d);
 }
+
 
 int graph_build(mm_ctx* c, hipStream_t cap) {
   const int kind = c->probe.kind;
@@ -2709,13 +2772,13 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
                     c->k64.xpolar(false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx,
                                   p->ny, p->nz, 0.0, 0.0, nullptr, nullptr) &&
                     c->k64.xelener(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx, p->ny,
-                                   p->nz, nullptr, nullptr, nullptr, nullptr, nullptr, &nb);
+                                   p->nz, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, &nb);
       else
         c->xpolar = c->k32.xpolar && c->k32.xelener &&
                     c->k32.xpolar(false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx,
                                   p->ny, p->nz, 0.0f, 0.0, nullptr, nullptr) &&
                     c->k32.xelener(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx, p->ny,
-                                   p->nz, nullptr, nullptr, nullptr, nullptr, nullptr, &nb);
+                                   p->nz, nullptr, nullptr, nullptr, nullptr, 0.0f, nullptr, nullptr, &nb);
       if (nb > kMaxPartBlocks) c->xpolar = false;
       nb = 0;
       if (precision == MM_FP64)
@@ -2916,7 +2979,23 @@ extern "C" int mm_field(mm_ctx* c, int32_t which, void** dev, int32_t* esz, int3
     case 0: p = c->phi; n = c->V; break;
     case 1: p = c->cons; n = c->E; break;
     case 2: p = c->el_en; n = 1; break;
-    case 3: p = c->fdiff; n = c->V; break;
+    case 3:
+      if (c->fd_split) {
+        // the last step passed f_diff + el_grad on as spectra: form it now
+        const size_t nn = (size_t)c->V * c->n3;
+        if (c->prec == MM_FP64)
+          hipLaunchKernelGGL(k_add_to<double>, dim3(blocks_for(nn)), dim3(kBlock), 0, c->s, (double*)c->fdiff,
+                             (const double*)c->elgrad, nn);
+        else
+          hipLaunchKernelGGL(k_add_to<float>, dim3(blocks_for(nn)), dim3(kBlock), 0, c->s, (float*)c->fdiff,
+                             (const float*)c->elgrad, nn);
+        MM_HIP(hipGetLastError());
+        MM_HIP(hipStreamSynchronize(c->s));
+        c->fd_split = false;
+      }
+      p = c->fdiff;
+      n = c->V;
+      break;
     case 4:
       // with the fused x path the strain lives as x half-spectra: form the
       // real fields now (valid until the next step)

@@ -183,6 +183,119 @@ template <typename T> struct OpGrad {
   template <int NW, class S> __device__ void reduce(const S&) const {}
 };
 
+// The gradient block with the kx and ky factors left to the inverse y pass
+// (k_y_pass_grad): in: phi_hat after the x and y passes (sphi[g]); the
+// z-transformed phi_hat is written back in place for the Allen-Cahn update.
+// out[3g] = phi_hat, out[3g+1] = phi_hat i kf_z, out[3g+2] = phi_hat G_zz kq_z.
+template <typename T> struct OpGrad3 {
+  static constexpr int NIN = 1, NOUT = 3, MAXT = 512, MINW = 4;
+  static constexpr int PLAN = 0;
+  const DevParams* P;
+  C2<T>* sphi;
+  C2<T>* work;
+  size_t nc;
+  const int* gate = nullptr;
+  template <int E> struct State { C2<T> s[E]; };
+  __device__ const C2<T>* input(int g, int) const { return sphi + (size_t)g * nc; }
+  __device__ C2<T>* output(int g, int o) const { return work + (size_t)(3 * g + o) * nc; }
+  template <int E>
+  __device__ void absorb(State<E>& st, int, int, const C2<T> (&v)[E], const ColK&, int, int) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) st.s[q] = v[q];
+  }
+  template <int E>
+  __device__ void finish(State<E>& st, int g, const ColK&, int col, int tl, int TPL, int ncols, bool valid) const {
+    if (!valid) return;
+    C2<T>* dst = sphi + (size_t)g * nc;
+#pragma unroll
+    for (int q = 0; q < E; ++q) dst[zcol(col, tl + TPL * q, P->nz)] = st.s[q];
+  }
+  template <int E>
+  __device__ void emit(const State<E>& st, int g, int o, C2<T> (&v)[E], const ColK&, int tl, int TPL) const {
+    const double gzz = P->gc[g * 9 + 8];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      double kf, kq;
+      kaxis(tl + TPL * q, P->nz, P->unit[2], kf, kq);
+      const double sr = (double)st.s[q].x, si = (double)st.s[q].y;
+      if (o == 0) v[q] = st.s[q];
+      else if (o == 1) v[q] = mk<T>((T)(-si * kf), (T)(sr * kf));
+      else v[q] = mk<T>((T)(sr * (gzz * kq)), (T)(si * (gzz * kq)));
+    }
+  }
+  template <int NW, class S> __device__ void reduce(const S&) const {}
+};
+
+// Inverse y pass of the gradient block (Z -> S) after OpGrad3: per line
+// (kx, z) of variant g = blockIdx.z, from A = phi_hat, C = phi_hat i kf_z and
+// Z = phi_hat G_zz kq_z (in + (3g + m) nc, Z layout) it forms the spectra of
+// Multi_MEUMAPPS.f:581-613 with the kx and ky factors applied here,
+//   grad_x = i kf_x A, grad_y = i kf_y A, grad_z = C,
+//   del.G del = a A + Z + i b C,
+//   a = G_xx kq_x + G_yy kq_y - (G_xy + G_yx) kf_x kf_y,
+//   b = (G_yz + G_zy) kf_y + (G_xz + G_zx) kf_x,
+// (so that the sum is phi_hat sum G_ij k_i k_j with kf_sq on the diagonal)
+// and transforms them along y into out + (4g + d) nc (S layout, the input of
+// k_x_chem).  Three spectra per variant cross HBM between the z and y passes
+// instead of four.
+template <typename T, class P>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS))
+k_y_pass_grad(const DevParams* __restrict__ Pp, const C2<T>* __restrict__ in, C2<T>* __restrict__ out, size_t nc,
+              const C2<T>* __restrict__ twL, int nz, int H, const int* __restrict__ gate) {
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  __shared__ C2<T> tw[L];
+  if (gated_off(gate)) return;
+  const int g = blockIdx.z, t = threadIdx.x;
+  for (int i = t; i < L; i += TileLM<T, P>::THREADS) tw[i] = twL[i];
+  __syncthreads();
+  const int b = t / TPL, tl = t % TPL, line = blockIdx.x * B + b;
+  const bool ok = line < nz * H;
+  const int kx = ok ? line / nz : 0, z = ok ? line % nz : 0;
+  const size_t srow = ((size_t)z * H + kx) * L;
+  const double* G = Pp->gc + g * 9;  // G[j*3+i] = grad_coeff(i,j)
+  double kfx, kqx;
+  kaxis(kx, Pp->nx, Pp->unit[0], kfx, kqx);
+  C2<T> A[E], C[E], Z[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const size_t zi = zidx(z, kx, tl + TPL * q, L, nz);
+    A[q] = in[(size_t)(3 * g) * nc + zi];
+    C[q] = in[(size_t)(3 * g + 1) * nc + zi];
+    Z[q] = in[(size_t)(3 * g + 2) * nc + zi];
+  }
+  IdxLineMajor idx{b * LP};
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    C2<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      double kfy, kqy;
+      kaxis(tl + TPL * q, Pp->ny, Pp->unit[1], kfy, kqy);
+      const double ar = (double)A[q].x, ai = (double)A[q].y;
+      if (d == 0) {
+        v[q] = mk<T>((T)(-ai * kfx), (T)(ar * kfx));
+      } else if (d == 1) {
+        v[q] = mk<T>((T)(-ai * kfy), (T)(ar * kfy));
+      } else if (d == 2) {
+        v[q] = C[q];
+      } else {
+        const double a = G[0] * kqx + G[4] * kqy + (G[3] + G[1]) * (-(kfx * kfy));
+        const double bb = (G[7] + G[5]) * kfy + (G[2] + G[6]) * kfx;
+        const double cr = (double)C[q].x, ci = (double)C[q].y;
+        v[q] = mk<T>((T)(a * ar + (double)Z[q].x - bb * ci), (T)(a * ai + (double)Z[q].y + bb * cr));
+      }
+    }
+    if (d > 0) line_sync<wave_lines<P>()>();
+    line_fft<T, +1, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, tw);
+    if (ok) {
+      C2<T>* dst = out + (size_t)(4 * g + d) * nc + srow;
+#pragma unroll
+      for (int q = 0; q < E; ++q) dst[tl + TPL * q] = v[q];
+    }
+  }
+}
+
 // --------------------------------------------------------- elastic iteration
 // in: stress-polarisation spectra x_1..x_6 after x and y (work[0..5]);
 // R = kf . x_hat, u = Q^-1 R (u(0) = 0); Parseval sums of |u - u_old|^2 and
@@ -388,6 +501,107 @@ template <typename T> struct OpCH {
   }
   template <int NW, class S> __device__ void reduce(const S&) const {}
 };
+
+// The same update with the x and y flux terms folded into the y pass
+// (k_y_pass_ch): in: work[2e] = c_hat + dt M (i kf_x F_x + i kf_y F_y),
+// work[2e+1] = F_z after x and y; out (work[2e]): c_hat new.
+template <typename T> struct OpCH2 {
+  static constexpr int NIN = 2, NOUT = 1, MAXT = 512, MINW = 4;
+  static constexpr int PLAN = 1;
+  const DevParams* P;
+  C2<T>* work;
+  size_t nc;
+  const int* gate = nullptr;
+  template <int E> struct State { double cr[E], ci[E], tr[E], ti[E]; };
+  __device__ const C2<T>* input(int g, int f) const { return work + (size_t)(2 * g + f) * nc; }
+  __device__ C2<T>* output(int g, int) const { return work + (size_t)(2 * g) * nc; }
+  template <int E>
+  __device__ void absorb(State<E>& st, int, int f, const C2<T> (&v)[E], const ColK&, int tl, int TPL) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      if (f == 0) {
+        st.cr[q] = (double)v[q].x;
+        st.ci[q] = (double)v[q].y;
+      } else {
+        double a, kq2;
+        kaxis(tl + TPL * q, P->nz, P->unit[2], a, kq2);
+        st.tr[q] = -a * (double)v[q].y;
+        st.ti[q] = a * (double)v[q].x;
+      }
+    }
+  }
+  template <int E>
+  __device__ void finish(State<E>&, int, const ColK&, int, int, int, int, bool) const {}
+  template <int E>
+  __device__ void emit(const State<E>& st, int g, int, C2<T> (&v)[E], const ColK& ck, int tl, int TPL) const {
+    const double dtm = P->dt * P->cmobs[g];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const KVec k = kfull(P, ck, tl + TPL * q);
+      const double den = 1.0 - dtm * (k.kq[0] + k.kq[1] + k.kq[2]);
+      v[q] = mk<T>((T)((st.cr[q] + dtm * st.tr[q]) / den), (T)((st.ci[q] + dtm * st.ti[q]) / den));
+      // MIXED stores c - cbar: its mean is zero, and the FP32 rounding the
+      // forward transforms leave in the k = 0 mode must not accumulate
+      if (sizeof(T) == 4 && k.kx == 0 && k.ky == 0 && k.kz == 0) v[q] = mk<T>(0, 0);
+    }
+  }
+  template <int NW, class S> __device__ void reduce(const S&) const {}
+};
+
+// Forward y pass of the Cahn-Hilliard update (S -> Z) with the x and y flux
+// terms of Multi_MEUMAPPS.f:924-943 folded in: per line (z, kx) of
+// composition e = blockIdx.z the x half-spectra c, F_x, F_y, F_z
+// (in + (4e + m) nc) are transformed along y, and
+//   N = c_hat + dt M_{e0+e} (i kf_x F_x_hat + i kf_y F_y_hat)
+// goes to out + 2e nc, F_z_hat to out + (2e + 1) nc.  OpCH2 then adds the z
+// term and divides by the semi-implicit denominator: two spectra cross HBM
+// between the passes instead of four.
+template <typename T, class P>
+__global__ void __launch_bounds__((TileLM<T, P>::THREADS))
+k_y_pass_ch(const DevParams* __restrict__ Pp, const C2<T>* __restrict__ in, C2<T>* __restrict__ out, size_t nc,
+            const C2<T>* __restrict__ twL, int nz, int H, int e0, const int* __restrict__ gate) {
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, B = TileLM<T, P>::B, LP = line_pitch<L>();
+  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  __shared__ C2<T> tw[L];
+  if (gated_off(gate)) return;
+  // e: the buffers' composition index; e0 + e that of the parameters
+  const int e = blockIdx.z, t = threadIdx.x;
+  for (int i = t; i < L; i += TileLM<T, P>::THREADS) tw[i] = twL[i];
+  __syncthreads();
+  const int b = t / TPL, tl = t % TPL, line = blockIdx.x * B + b;
+  const bool ok = line < nz * H;
+  const int kx = ok ? line / nz : 0, z = ok ? line % nz : 0;
+  const size_t srow = ((size_t)z * H + kx) * L;
+  double kfx, kqx;
+  kaxis(kx, Pp->nx, Pp->unit[0], kfx, kqx);
+  const double dtm = Pp->dt * Pp->cmobs[e0 + e];
+  IdxLineMajor idx{b * LP};
+  double ar[E], ai[E];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const C2<T>* src = in + (size_t)(4 * e + m) * nc + srow;
+    C2<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = ok ? src[tl + TPL * q] : mk<T>(0, 0);
+    if (m > 0) line_sync<wave_lines<P>()>();
+    line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, tw);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int ky = tl + TPL * q;
+      if (m == 0) {
+        ar[q] = (double)v[q].x;
+        ai[q] = (double)v[q].y;
+      } else if (m < 3) {
+        double a = kfx, kq2;
+        if (m == 2) kaxis(ky, Pp->ny, Pp->unit[1], a, kq2);
+        ar[q] += dtm * (-a * (double)v[q].y);
+        ai[q] += dtm * (a * (double)v[q].x);
+      }
+      if (ok && m == 2) out[(size_t)(2 * e) * nc + zidx(z, kx, ky, L, nz)] = mk<T>((T)ar[q], (T)ai[q]);
+      if (ok && m == 3) out[(size_t)(2 * e + 1) * nc + zidx(z, kx, ky, L, nz)] = v[q];
+    }
+  }
+}
 
 // -------------------------------------------------------------- Allen-Cahn
 // in: work[v] = f_diff after x and y; reads the full phi_hat (sphi[v]);

@@ -43,6 +43,56 @@ int launch_zf_elastic(const Fft3& f, const OpElastic<T>& op, int v, hipStream_t 
 template int launch_zf_elastic<double>(const Fft3&, const OpElastic<double>&, int, hipStream_t, int, double, int*);
 template int launch_zf_elastic<float>(const Fft3&, const OpElastic<float>&, int, hipStream_t, int, double, int*);
 
+// the y pass of the Cahn-Hilliard update with the x and y flux terms folded in
+template <typename T>
+int launch_y_ch(const Fft3& f, const DevParams* P, const void* in, void* out, int ne, int e0, hipStream_t s) {
+  const int H = f.H, nz = f.nz;
+  if (f.probe) f.probe->begin(MM_PROBE_Y, s);
+  switch (f.ny) {
+#define X(L_)                                                                                             \
+  case L_: {                                                                                              \
+    using TL = TileLM<T, Plan<L_>>;                                                                       \
+    dim3 grid((nz * H + TL::B - 1) / TL::B, 1, ne);                                                       \
+    hipLaunchKernelGGL((k_y_pass_ch<T, Plan<L_>>), grid, dim3(TL::THREADS), 0, s, P, (const C2<T>*)in,    \
+                       (C2<T>*)out, f.ncplx(), (const C2<T>*)f.twNy, nz, H, e0, f.gate);                  \
+    break;                                                                                                \
+  }
+    X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+#undef X
+    default: return MM_ERR_ARG;
+  }
+  if (f.probe) f.probe->end(MM_PROBE_Y, s, (double)ne * 6.0 * f.ncplx() * 2 * sizeof(T));
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+template int launch_y_ch<double>(const Fft3&, const DevParams*, const void*, void*, int, int, hipStream_t);
+template int launch_y_ch<float>(const Fft3&, const DevParams*, const void*, void*, int, int, hipStream_t);
+
+// the inverse y pass of the gradient block after OpGrad3 (nv variants)
+template <typename T>
+int launch_y_grad(const Fft3& f, const DevParams* P, const void* in, void* out, int nv, hipStream_t s) {
+  const int H = f.H, nz = f.nz;
+  if (f.probe) f.probe->begin(MM_PROBE_Y, s);
+  switch (f.ny) {
+#define X(L_)                                                                                             \
+  case L_: {                                                                                              \
+    using TL = TileLM<T, PlanLean<L_>>;                                                                   \
+    dim3 grid((nz * H + TL::B - 1) / TL::B, 1, nv);                                                       \
+    hipLaunchKernelGGL((k_y_pass_grad<T, PlanLean<L_>>), grid, dim3(TL::THREADS), 0, s, P,                \
+                       (const C2<T>*)in, (C2<T>*)out, f.ncplx(), (const C2<T>*)f.twNy, nz, H, f.gate);    \
+    break;                                                                                                \
+  }
+    X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+#undef X
+    default: return MM_ERR_ARG;
+  }
+  if (f.probe) f.probe->end(MM_PROBE_Y, s, (double)nv * 7.0 * f.ncplx() * 2 * sizeof(T));
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+template int launch_y_grad<double>(const Fft3&, const DevParams*, const void*, void*, int, hipStream_t);
+template int launch_y_grad<float>(const Fft3&, const DevParams*, const void*, void*, int, hipStream_t);
+
 #define INST(T_, OP_) \
   template int launch_zfused<T_, OP_<T_>>(const Fft3&, const OP_<T_>&, int, hipStream_t, int, double, int*);
 INST(double, OpGrad)
@@ -51,6 +101,10 @@ INST(double, OpElastic)
 INST(float, OpElastic)
 INST(double, OpCH)
 INST(float, OpCH)
+INST(double, OpCH2)
+INST(float, OpCH2)
+INST(double, OpGrad3)
+INST(float, OpGrad3)
 INST(double, OpAC)
 INST(float, OpAC)
 #undef INST
