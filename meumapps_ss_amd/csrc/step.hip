@@ -823,25 +823,23 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
            size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T* __restrict__ el_en,
            T* __restrict__ elgrad, T* __restrict__ fdiff, C2<T>* __restrict__ sfd, T scale,
            double* __restrict__ part, const int* __restrict__ gate) {
-  constexpr int L = P::L, TPL = P::TPL, E = P::E, NW = (TileLM<T, P>::THREADS + 63) / 64;
+  constexpr int L = P::L, TPL = P::TPL, E = P::E, NT = TileLM<T, P>::THREADS, NW = (NT + 63) / 64;
   __shared__ C2<T> lds[TileLM<T, P>::LDS];
   __shared__ C2<T> twl[L], twn[2 * L];
   __shared__ double sh[NW];
+  // XS: el_grad of this thread's points, kept in LDS (slot (v E + q) NT + t)
+  // until the r2c transforms add it to f_diff: registers hold the strain
+  __shared__ C2<T> fdl[XS ? V * E * NT : 1];
   if (gated_off(gate)) return;
   const XRows<T, P> w(ny);
-  for (int e = w.t; e < L; e += TileLM<T, P>::THREADS) twl[e] = twL[e];
-  for (int e = w.t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
+  for (int e = w.t; e < L; e += NT) twl[e] = twL[e];
+  for (int e = w.t; e < 2 * L; e += NT) twn[e] = twN[e];
   __syncthreads();
   C2<T> s[6][E];
   x_c2r_n<T, P, 6>(w, sstrain, nc, ny, lds, twl, twn, s);
   // rows past ny read row ny-1 and store nothing
   const size_t row = ((size_t)w.z * ny + (w.yl < ny ? w.yl : ny - 1)) * (size_t)(2 * L);
   double s_el = 0.0;
-  C2<T> fx[XS ? V : 1][E];  // XS: f_diff of this thread's points for the r2c
-#pragma unroll
-  for (int v = 0; v < (XS ? V : 1); ++v)
-#pragma unroll
-    for (int q = 0; q < E; ++q) fx[v][q] = mk<T>(0, 0);
   if (w.yl < ny) {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -850,7 +848,8 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         ph[v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[n];
-        fd[v] = ((const C2<T>*)(fdiff + (size_t)v * n3 + row))[n];
+        // XS adds the chemistry's f_diff at the r2c (binary64 only: the same sum)
+        fd[v] = XS ? mk<T>(0, 0) : ((const C2<T>*)(fdiff + (size_t)v * n3 + row))[n];
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -873,14 +872,25 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         ((C2<T>*)(elgrad + (size_t)v * n3 + row))[n] = go[v];
-        if constexpr (XS) fx[v][q] = fd[v];
+        if constexpr (XS) fdl[(v * E + q) * NT + w.t] = go[v];
         else ((C2<T>*)(fdiff + (size_t)v * n3 + row))[n] = fd[v];
       }
     }
   }
   if constexpr (XS) {
+    // one variant at a time (not unrolled: the loads of the next would be
+    // hoisted into the transform of this one)
+#pragma unroll 1
+    for (int v = 0; v < V; ++v) {
+      C2<T> x[E];
 #pragma unroll
-    for (int v = 0; v < V; ++v) x_r2c_store<T, P>(w, fx[v], lds, twl, twn, sfd + (size_t)v * nc, ny, scale);
+      for (int q = 0; q < E; ++q) {
+        const C2<T> fc = ((const C2<T>*)(fdiff + (size_t)v * n3 + row))[w.tl + TPL * q];
+        const C2<T> g = fdl[(v * E + q) * NT + w.t];
+        x[q] = mk<T>((T)((double)fc.x + (double)g.x), (T)((double)fc.y + (double)g.y));
+      }
+      x_r2c_store<T, P>(w, x, lds, twl, twn, sfd + (size_t)v * nc, ny, scale);
+    }
   }
   const double rsum = block_reduce<0, NW>(s_el, sh);
   if (w.t == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = rsum;
