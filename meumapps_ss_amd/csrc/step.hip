@@ -2282,10 +2282,11 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   // hands f_diff to the Allen-Cahn update as x half-spectra in the stage (the
   // real f_diff + el_grad is not formed; fdiff keeps the chemistry's part and
   // mm_field adds el_grad on demand).  Noise steps and recorded steps need the
-  // real f_diff.  Binary64 only: the binary32 kernel (8 points per thread)
-  // spills with the three extra transforms and runs slower than the separate
-  // x pass (0.32 -> 0.55 ms at 256^3), the binary64 one saves 0.06 ms.
-  const bool xs = sizeof(T) == 8 && elastic && c->xpolar && c->zfused && c->ch_done &&
+  // real f_diff.  Binary64 rows of up to 256 points only: the binary32 kernel
+  // (8 points per thread) spills with the three extra transforms and runs
+  // slower than the separate x pass (0.32 -> 0.55 ms at 256^3), and so does
+  // the binary64 one on 512-point rows (5.4 -> 7.0 ms at 512^3).
+  const bool xs = sizeof(T) == 8 && c->fft.nx <= 256 && elastic && c->xpolar && c->zfused && c->ch_done &&
                   c->step > c->prm.noise_step && !rec_fd;
   c->xs_tail = xs;
   c->fd_split = xs;
