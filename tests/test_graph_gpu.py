@@ -49,6 +49,11 @@ def test_graph_replays_match_eager(gpu_lib, name, prec, iters):
             re_, rg = e.step(1)[0], g.step(1)[0]
             _rows_same(rg, re_, f"{name} step {k + 1}")
             _same(g.download(), e.download(), f"{name} step {k + 1}")
+            # the intermediates of the replayed step (f_diff is formed on demand
+            # where elener hands it to the Allen-Cahn update as spectra)
+            for f in ("el_grad", "f_diff", "el_en"):
+                a, b = g.field(f), e.field(f)
+                assert np.array_equal(a, b), f"{name} step {k + 1}: {f} max {np.abs(a - b).max():.3e}"
 
 
 def test_graph_off_again_and_restart(gpu_lib):
