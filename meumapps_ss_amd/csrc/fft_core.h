@@ -212,12 +212,52 @@ template <bool WAVE> __device__ __forceinline__ void line_sync() {
   }
 }
 
+// Twiddle sources of the Stockham stages.  TwTable reads the forward table
+// exp(-2 pi i e / L), e in [0, L) (global or LDS) at every use; TwRegs holds the
+// factors this thread's butterflies use in every stage after the first (NB (R-1)
+// per stage), loaded once, for kernels that run many transforms of one length.
+template <typename T> struct TwTable {
+  const C2<T>* __restrict__ p;
+  template <int S> __device__ __forceinline__ C2<T> get(int, int, int e) const { return p[e]; }
+};
+template <class P> __host__ __device__ constexpr int tw_count_upto(int S) {
+  int n = 0;
+  for (int s = 1; s < S && s < P::NS; ++s) {
+    const int R = s == 1 ? P::R1 : s == 2 ? P::R2 : s == 3 ? P::R3 : P::R4;
+    n += (P::E / R) * (R - 1);
+  }
+  return n;
+}
+template <typename T, class P> struct TwRegs {
+  static constexpr int N = tw_count_upto<P>(P::NS) > 0 ? tw_count_upto<P>(P::NS) : 1;
+  C2<T> w[N];
+  template <int S> __device__ __forceinline__ C2<T> get(int i, int m, int) const {
+    constexpr int R = S == 1 ? P::R1 : S == 2 ? P::R2 : S == 3 ? P::R3 : P::R4;
+    return w[tw_count_upto<P>(S) + i * (R - 1) + (m - 1)];
+  }
+  template <int S> __device__ __forceinline__ void load_stage(const C2<T>* __restrict__ tw, int tl) {
+    if constexpr (S < P::NS) {
+      constexpr int R = S == 1 ? P::R1 : S == 2 ? P::R2 : S == 3 ? P::R3 : P::R4;
+      constexpr int Ls = S == 1 ? P::R0 : S == 2 ? P::R0 * P::R1 : S == 3 ? P::R0 * P::R1 * P::R2
+                                                                        : P::R0 * P::R1 * P::R2 * P::R3;
+#pragma unroll
+      for (int i = 0; i < P::E / R; ++i) {
+        const int k = (tl + P::TPL * i) % Ls;
+#pragma unroll
+        for (int m = 1; m < R; ++m) w[tw_count_upto<P>(S) + i * (R - 1) + (m - 1)] = tw[m * k * (P::L / (Ls * R))];
+      }
+      load_stage<S + 1>(tw, tl);
+    }
+  }
+  __device__ __forceinline__ void load(const C2<T>* __restrict__ tw, int tl) { load_stage<1>(tw, tl); }
+};
+
 // One Stockham stage. `idx(e)` maps a line position to an LDS slot of this
-// thread's line; tw is the forward table exp(-2 pi i e / L), e in [0, L).
+// thread's line; tw is a twiddle source (TwTable of the forward table
+// exp(-2 pi i e / L), e in [0, L), or TwRegs).
 // WAVE: the lines are held by whole wavefronts (wave_lines<P>()).
-template <typename T, int DIR, class P, int S, class Idx, bool WAVE = false>
-__device__ __forceinline__ void stage(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx,
-                                      const C2<T>* __restrict__ tw) {
+template <typename T, int DIR, class P, int S, class Idx, bool WAVE, class TW>
+__device__ __forceinline__ void stage(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx, const TW& tw) {
   constexpr int R = StageR<P, S>::R;
   constexpr int Ls = StageR<P, S>::Ls;
   constexpr int NB = P::E / R;  // butterflies per thread
@@ -239,7 +279,7 @@ __device__ __forceinline__ void stage(C2<T> (&v)[P::E], int tl, C2<T>* lds, cons
     if constexpr (S > 0) {
 #pragma unroll
       for (int m = 1; m < R; ++m) {
-        C2<T> w = tw[m * k * (L / (Ls * R))];
+        C2<T> w = tw.template get<S>(i, m, m * k * (L / (Ls * R)));
         if constexpr (DIR > 0) w.y = -w.y;
         a[m] = cmul<T>(a[m], w);
       }
@@ -257,12 +297,11 @@ __device__ __forceinline__ void stage(C2<T> (&v)[P::E], int tl, C2<T>* lds, cons
   if constexpr (!last) line_sync<WAVE>();
 }
 
-template <typename T, int DIR, class P, int S, class Idx, bool WAVE = false>
-__device__ __forceinline__ void stages(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx,
-                                       const C2<T>* __restrict__ tw) {
+template <typename T, int DIR, class P, int S, class Idx, bool WAVE, class TW>
+__device__ __forceinline__ void stages(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx, const TW& tw) {
   if constexpr (S < P::NS) {
-    stage<T, DIR, P, S, Idx, WAVE>(v, tl, lds, idx, tw);
-    stages<T, DIR, P, S + 1, Idx, WAVE>(v, tl, lds, idx, tw);
+    stage<T, DIR, P, S, Idx, WAVE, TW>(v, tl, lds, idx, tw);
+    stages<T, DIR, P, S + 1, Idx, WAVE, TW>(v, tl, lds, idx, tw);
   }
 }
 
@@ -274,7 +313,13 @@ __device__ __forceinline__ void stages(C2<T> (&v)[P::E], int tl, C2<T>* lds, con
 template <typename T, int DIR, class P, class Idx, bool WAVE = false>
 __device__ __forceinline__ void line_fft(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx,
                                          const C2<T>* __restrict__ tw) {
-  stages<T, DIR, P, 0, Idx, WAVE>(v, tl, lds, idx, tw);
+  stages<T, DIR, P, 0, Idx, WAVE, TwTable<T>>(v, tl, lds, idx, TwTable<T>{tw});
+}
+// the same with the stage twiddles held in registers (rt.load(table, tl) first)
+template <typename T, int DIR, class P, class Idx, bool WAVE = false>
+__device__ __forceinline__ void line_fft(C2<T> (&v)[P::E], int tl, C2<T>* lds, const Idx& idx,
+                                         const TwRegs<T, P>& rt) {
+  stages<T, DIR, P, 0, Idx, WAVE, TwRegs<T, P>>(v, tl, lds, idx, rt);
 }
 
 // LDS slot maps

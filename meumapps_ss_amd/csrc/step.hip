@@ -575,9 +575,10 @@ __device__ __forceinline__ void x_c2r_pipe(LD& load, CV& c2r, X& xa, X& xb) {
 // The loads of component m+1 are issued before component m is transformed;
 // they are unconditional (indices clamped into the array; rows past ny are
 // computed but never stored), so that no load waits behind a branch.
-template <typename T, class P, int NC>
+// twl: the stage twiddles, the LDS table or a TwRegs set
+template <typename T, class P, int NC, class TWS>
 __device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, size_t nc, int ny, C2<T>* lds,
-                                        const C2<T>* twl, const C2<T>* twn, C2<T> (&s)[NC][P::E]) {
+                                        const TWS& twl, const C2<T>* twn, C2<T> (&s)[NC][P::E]) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
   const IdxLineMajor idx{w.bl * LP};
   const size_t base = (size_t)w.z * (L + 1) * ny + (size_t)(w.yg < ny ? w.yg : ny - 1);
@@ -655,7 +656,13 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
   for (int e = t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
   __syncthreads();
   const IdxLineMajor idx{w.bl * LP};
-  // phi of this thread's points, loaded ahead of the transforms
+  // the stage twiddles of this thread, in registers for all twelve transforms
+  TwRegs<T, P> trw;
+  trw.load(twl, tl);
+  C2<T> s[6][E];
+  // 1. c2r of the six strain components (in; may be the same buffer as work)
+  x_c2r_n<T, P, 6>(w, in, nc, ny, lds, trw, twn, s);
+  // phi of this thread's points
   C2<T> ph[E][V];
   {
     const size_t row = ((size_t)z * ny + (yl < ny ? yl : ny - 1)) * (size_t)(2 * L);
@@ -664,9 +671,6 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
 #pragma unroll
       for (int v = 0; v < V; ++v) ph[q][v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[tl + TPL * q];
   }
-  C2<T> s[6][E];
-  // 1. c2r of the six strain components (in; may be the same buffer as work)
-  x_c2r_n<T, P, 6>(w, in, nc, ny, lds, twl, twn, s);
   // 2. polarization stress at the 2E points of this thread
   if (yl < ny) {
     const double cfac = cfac_dev ? *cfac_dev : cfac_arg;
@@ -697,7 +701,7 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = s[m][q];
     __syncthreads();
-    line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, twl);
+    line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, trw);
     if constexpr (P::NS > 1) line_sync<wave_lines<P>()>();
 #pragma unroll
     for (int q = 0; q < E; ++q) lds[idx(tl + TPL * q)] = v[q];
