@@ -575,10 +575,25 @@ __device__ __forceinline__ void x_c2r_pipe(LD& load, CV& c2r, X& xa, X& xb) {
 // The loads of component m+1 are issued before component m is transformed;
 // they are unconditional (indices clamped into the array; rows past ny are
 // computed but never stored), so that no load waits behind a branch.
-// twl: the stage twiddles, the LDS table or a TwRegs set
-template <typename T, class P, int NC, class TWS>
-__device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, size_t nc, int ny, C2<T>* lds,
-                                        const TWS& twl, const C2<T>* twn, C2<T> (&s)[NC][P::E]) {
+// Untangle factors exp(-2 pi i k / Nx) of the x c2r, k = tl + TPL q: read from
+// the table at every use (UntTable) or held in registers (UntRegs).
+template <typename T> struct UntTable {
+  const C2<T>* p;
+  __device__ __forceinline__ C2<T> get(int, int k) const { return p[k]; }
+};
+template <typename T, int E> struct UntRegs {
+  C2<T> w[E];
+  __device__ __forceinline__ C2<T> get(int q, int) const { return w[q]; }
+};
+
+// twl: the stage twiddles, the LDS table or a TwRegs set; twn: the untangle
+// factors, a table pointer or an UntRegs set.  NBUF = 2: lds0 holds two row
+// buffers and component m uses buffer m mod 2; the previous user of that
+// buffer, component m - 2, finished before every thread passed the barrier
+// after component m - 1's gather, so no barrier is needed before the gather.
+template <typename T, class P, int NC, class TWS, class UN, int NBUF = 1>
+__device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, size_t nc, int ny, C2<T>* lds0,
+                                        const TWS& twl, const UN& twn, C2<T> (&s)[NC][P::E]) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
   const IdxLineMajor idx{w.bl * LP};
   const size_t base = (size_t)w.z * (L + 1) * ny + (size_t)(w.yg < ny ? w.yg : ny - 1);
@@ -594,7 +609,8 @@ __device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, s
     }
   };
   auto c2r = [&](int m, const C2<T>(&x)[E + 1]) {
-    if (m > 0) __syncthreads();
+    C2<T>* lds = lds0 + (NBUF == 2 ? (m & 1) * TileLM<T, P>::LDS : 0);
+    if (NBUF == 1 && m > 0) __syncthreads();
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int kx = w.r + TPL * i;
@@ -616,7 +632,9 @@ __device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, s
       if (k == 0) { xk.y = 0; xc.y = 0; }
       C2<T> ev = mk<T>(xk.x + xc.x, xk.y - xc.y);
       C2<T> dd = mk<T>(xk.x - xc.x, xk.y + xc.y);
-      C2<T> tw = twn[k];
+      C2<T> tw;
+      if constexpr (std::is_pointer<UN>::value || std::is_array<UN>::value) tw = twn[k];
+      else tw = twn.get(q, k);
       tw.y = -tw.y;
       C2<T> od = cmul<T>(dd, tw);
       v[q] = mk<T>(ev.x - od.y, ev.y + od.x);
@@ -646,8 +664,9 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
           size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T scale, double cfac_arg,
           const double* __restrict__ cfac_dev, const int* __restrict__ gate) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
-  __shared__ C2<T> lds[TileLM<T, P>::LDS];
-  // twiddle tables staged in LDS: the 12 transforms read them at every stage
+  // two row buffers: component m of a phase works in buffer m mod 2
+  __shared__ C2<T> ldsb[2 * TileLM<T, P>::LDS];
+  // twiddle tables staged in LDS (then held in registers)
   __shared__ C2<T> twl[L], twn[2 * L];
   if (gated_off(gate)) return;
   const XRows<T, P> w(ny);
@@ -659,9 +678,12 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
   // the stage twiddles of this thread, in registers for all twelve transforms
   TwRegs<T, P> trw;
   trw.load(twl, tl);
+  UntRegs<T, E> ur;
+#pragma unroll
+  for (int q = 0; q < E; ++q) ur.w[q] = twn[tl + TPL * q];
   C2<T> s[6][E];
   // 1. c2r of the six strain components (in; may be the same buffer as work)
-  x_c2r_n<T, P, 6>(w, in, nc, ny, lds, trw, twn, s);
+  x_c2r_n<T, P, 6, TwRegs<T, P>, UntRegs<T, E>, 2>(w, in, nc, ny, ldsb, trw, ur, s);
   // phi of this thread's points
   C2<T> ph[E][V];
   {
@@ -693,14 +715,23 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
       for (int m = 0; m < 6; ++m) s[m][q].y = (T)xo[m];
     }
   }
-  // 3. r2c of the six stress components, scaled, written back in place
+  // 3. r2c of the six stress components, scaled, written back in place; the
+  // untangle factors of this thread's kx (gather view) in registers
   const T hs = (T)0.5 * scale;
+  C2<T> wn[E + 1];
+#pragma unroll
+  for (int i = 0; i <= E; ++i) {
+    const int kx = r + TPL * i;
+    wn[i] = twn[kx <= L ? kx : L];
+  }
 #pragma unroll
   for (int m = 0; m < 6; ++m) {
+    // buffer m mod 2: its previous user (component m - 2, or the strain
+    // transforms) finished before the barrier of component m - 1
+    C2<T>* lds = ldsb + (m & 1) * TileLM<T, P>::LDS;
     C2<T> v[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = s[m][q];
-    __syncthreads();
     line_fft<T, -1, P, IdxLineMajor, wave_lines<P>()>(v, tl, lds, idx, trw);
     if constexpr (P::NS > 1) line_sync<wave_lines<P>()>();
 #pragma unroll
@@ -717,8 +748,7 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
         C2<T> zc = lds[bg * LP + k2 + (k2 >> 4)];
         C2<T> ev = mk<T>(zk.x + zc.x, zk.y - zc.y);
         C2<T> od = mk<T>(zk.y + zc.y, -(zk.x - zc.x));
-        C2<T> w = twn[kx];
-        C2<T> xx = cadd<T>(ev, cmul<T>(w, od));
+        C2<T> xx = cadd<T>(ev, cmul<T>(wn[i], od));
         out[((size_t)z * (L + 1) + kx) * ny + yg] = cscale<T>(xx, hs);
       }
     }
