@@ -666,21 +666,18 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
   constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
   // two row buffers: component m of a phase works in buffer m mod 2
   __shared__ C2<T> ldsb[2 * TileLM<T, P>::LDS];
-  // twiddle tables staged in LDS (then held in registers)
-  __shared__ C2<T> twl[L], twn[2 * L];
   if (gated_off(gate)) return;
   const XRows<T, P> w(ny);
   const int z = w.z, t = w.t, tl = w.tl, yl = w.yl, bg = w.bg, r = w.r, yg = w.yg;
-  for (int e = t; e < L; e += TileLM<T, P>::THREADS) twl[e] = twL[e];
-  for (int e = t; e < 2 * L; e += TileLM<T, P>::THREADS) twn[e] = twN[e];
-  __syncthreads();
+  (void)t;
   const IdxLineMajor idx{w.bl * LP};
-  // the stage twiddles of this thread, in registers for all twelve transforms
+  // the twiddles of this thread (stages, and the untangle factors of the
+  // c2r and r2c), in registers for all twelve transforms
   TwRegs<T, P> trw;
-  trw.load(twl, tl);
+  trw.load(twL, tl);
   UntRegs<T, E> ur;
 #pragma unroll
-  for (int q = 0; q < E; ++q) ur.w[q] = twn[tl + TPL * q];
+  for (int q = 0; q < E; ++q) ur.w[q] = twN[tl + TPL * q];
   C2<T> s[6][E];
   // 1. c2r of the six strain components (in; may be the same buffer as work)
   x_c2r_n<T, P, 6, TwRegs<T, P>, UntRegs<T, E>, 2>(w, in, nc, ny, ldsb, trw, ur, s);
@@ -722,7 +719,7 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
 #pragma unroll
   for (int i = 0; i <= E; ++i) {
     const int kx = r + TPL * i;
-    wn[i] = twn[kx <= L ? kx : L];
+    wn[i] = twN[kx <= L ? kx : L];
   }
 #pragma unroll
   for (int m = 0; m < 6; ++m) {
@@ -858,7 +855,9 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
            T* __restrict__ elgrad, T* __restrict__ fdiff, C2<T>* __restrict__ sfd, T scale,
            double* __restrict__ part, const int* __restrict__ gate) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, NT = TileLM<T, P>::THREADS, NW = (NT + 63) / 64;
-  __shared__ C2<T> lds[TileLM<T, P>::LDS];
+  // row buffers: two (used in turn by the strain components) unless XS
+  __shared__ C2<T> ldsb[(XS ? 1 : 2) * TileLM<T, P>::LDS];
+  C2<T>* const lds = ldsb;
   __shared__ C2<T> twl[L], twn[2 * L];
   __shared__ double sh[NW];
   // XS: el_grad of this thread's points, kept in LDS (slot (v E + q) NT + t)
@@ -870,7 +869,8 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
   for (int e = w.t; e < 2 * L; e += NT) twn[e] = twN[e];
   __syncthreads();
   C2<T> s[6][E];
-  x_c2r_n<T, P, 6>(w, sstrain, nc, ny, lds, twl, twn, s);
+  if constexpr (XS) x_c2r_n<T, P, 6>(w, sstrain, nc, ny, lds, twl, twn, s);
+  else x_c2r_n<T, P, 6, C2<T>[L], C2<T>[2 * L], 2>(w, sstrain, nc, ny, ldsb, twl, twn, s);
   // rows past ny read row ny-1 and store nothing
   const size_t row = ((size_t)w.z * ny + (w.yl < ny ? w.yl : ny - 1)) * (size_t)(2 * L);
   double s_el = 0.0;
