@@ -2855,6 +2855,11 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
         if (nb > kMaxPartBlocks || 4 * c->E > kStageN || 4 * c->V > kMaxBatch) c->xchem = false;
       }
     }
+    // binary32: each column of the elastic z kernel held by whole wavefronts
+    // (its LDS exchanges without workgroup barriers): 0.34 -> 0.32 ms per
+    // launch at 256^3; binary64 keeps the interleaved columns (0.47 -> 0.56 ms
+    // with wavefront columns)
+    c->zf_el_var = precision == MM_MIXED ? 2 : 0;
     const char* zv = std::getenv("MM_ZF_ELASTIC_VARIANT");
     if (zv) c->zf_el_var = std::max(0, std::min(kZfElasticVariants - 1, std::atoi(zv)));
   }

@@ -13,6 +13,8 @@
 //   OpCH       Cahn-Hilliard       Multi_MEUMAPPS.f:912-947   4 in  -> 1 out
 //   OpAC       Allen-Cahn          Multi_MEUMAPPS.f:958-997   1 in  -> 1 out (reads phi_hat)
 #pragma once
+#include <type_traits>
+
 #include "fft_pass.h"
 #include "step_kernels.h"
 
@@ -80,15 +82,25 @@ k_z_fused(Op op, const DevParams* __restrict__ Pr, const C2<T>* __restrict__ twL
   if (gated_off(op.gate)) return;
   constexpr int TPL = P::TPL, E = P::E, B = TL::B;
   static_assert(B % kTW == 0, "a workgroup owns whole tiles");
-  __shared__ C2<T> lds[TL::LDS];
+  // WL: each column is held by whole wavefronts (thread t works on column
+  // t / TPL), so the LDS exchanges of a transform need only wavefront order;
+  // otherwise the columns of a tile are interleaved thread by thread
+  // (element-major LDS), and every exchange is a workgroup barrier
+  constexpr bool WL = Op::WL && wave_lines<P>();
+  using Idx = std::conditional_t<WL, IdxLineMajor, IdxElemMajor>;
+  __shared__ C2<T> lds[WL ? B * line_pitch<L>() : TL::LDS];
   __shared__ C2<T> tw[L];
-  const int t = threadIdx.x, b = t % B, tl = t / B, col = blockIdx.x * B + b, g = blockIdx.y;
+  const int t = threadIdx.x, b = WL ? t / TPL : t % B, tl = WL ? t % TPL : t / B;
+  const int col = blockIdx.x * B + b, g = blockIdx.y;
   for (int e = t; e < L; e += TL::THREADS) tw[e] = twL[e];
   __syncthreads();
   const bool valid = col < ncols;
   const int colc = valid ? col : 0;
   const ColK ck = colk(op.P, colc);
-  IdxElemMajor idx{b, B};
+  const Idx idx = [&]() {
+    if constexpr (WL) return IdxLineMajor{b * line_pitch<L>()};
+    else return IdxElemMajor{b, B};
+  }();
   typename Op::template State<E> st;
   // the column of input f+1 is loaded while input f is transformed
   C2<T> vn[E];
@@ -107,8 +119,8 @@ k_z_fused(Op op, const DevParams* __restrict__ Pr, const C2<T>* __restrict__ twL
 #pragma unroll
       for (int q = 0; q < E; ++q) vn[q] = in[zcol(colc, tl + TPL * q, L)];
     }
-    if (f > 0) __syncthreads();
-    line_fft<T, -1, P>(v, tl, lds, idx, tw);
+    if (f > 0) line_sync<WL>();
+    line_fft<T, -1, P, Idx, WL>(v, tl, lds, idx, tw);
     op.template absorb<E>(st, g, f, v, ck, tl, TPL);
   }
   op.template finish<E>(st, g, ck, colc, tl, TPL, ncols, valid);
@@ -116,8 +128,8 @@ k_z_fused(Op op, const DevParams* __restrict__ Pr, const C2<T>* __restrict__ twL
   for (int o = 0; o < Op::NOUT; ++o) {
     C2<T> v[E];
     op.template emit<E>(st, g, o, v, ck, tl, TPL);
-    __syncthreads();
-    line_fft<T, +1, P>(v, tl, lds, idx, tw);
+    line_sync<WL>();
+    line_fft<T, +1, P, Idx, WL>(v, tl, lds, idx, tw);
     C2<T>* out = op.output(g, o);
     if (valid) {
 #pragma unroll
@@ -134,6 +146,7 @@ k_z_fused(Op op, const DevParams* __restrict__ Pr, const C2<T>* __restrict__ twL
 template <typename T> struct OpGrad {
   static constexpr int NIN = 1, NOUT = 4, MAXT = 512, MINW = 4;
   static constexpr int PLAN = 0;
+  static constexpr bool WL = false;
   const DevParams* P;
   C2<T>* sphi;
   C2<T>* work;
@@ -190,6 +203,7 @@ template <typename T> struct OpGrad {
 template <typename T> struct OpGrad3 {
   static constexpr int NIN = 1, NOUT = 3, MAXT = 512, MINW = 4;
   static constexpr int PLAN = 0;
+  static constexpr bool WL = false;
   const DevParams* P;
   C2<T>* sphi;
   C2<T>* work;
@@ -303,6 +317,7 @@ k_y_pass_grad(const DevParams* __restrict__ Pp, const C2<T>* __restrict__ in, C2
 template <typename T> struct OpElastic {
   static constexpr int NIN = 6, NOUT = 6, MAXT = 256, MINW = 2;
   static constexpr int PLAN = 1;
+  static constexpr bool WL = false;
   const DevParams* P;
   C2<T>* work;
   C2<T>* uhat;
@@ -438,18 +453,21 @@ template <typename T> struct OpElastic {
 };
 
 // launch-shape variants of the elastic op (plan kind, workgroup size cap,
-// minimum waves per SIMD), chosen at context creation by timing
-template <typename T, int PLAN_, int MAXT_, int MINW_> struct OpElasticV : OpElastic<T> {
+// minimum waves per SIMD, columns held by whole wavefronts), chosen at
+// context creation by precision
+template <typename T, int PLAN_, int MAXT_, int MINW_, bool WL_ = false> struct OpElasticV : OpElastic<T> {
   static constexpr int PLAN = PLAN_, MAXT = MAXT_, MINW = MINW_;
+  static constexpr bool WL = WL_;
   OpElasticV(const OpElastic<T>& b) : OpElastic<T>(b) {}
 };
-constexpr int kZfElasticVariants = 2;
+constexpr int kZfElasticVariants = 3;
 
 // ------------------------------------------------------------ Cahn-Hilliard
 // in: work[4e] = c, work[4e+1..3] = Fx, Fy, Fz after x and y; out: c_hat new.
 template <typename T> struct OpCH {
   static constexpr int NIN = 4, NOUT = 1, MAXT = 512, MINW = 4;
   static constexpr int PLAN = 1;
+  static constexpr bool WL = false;
   const DevParams* P;
   C2<T>* work;
   size_t nc;
@@ -508,6 +526,7 @@ template <typename T> struct OpCH {
 template <typename T> struct OpCH2 {
   static constexpr int NIN = 2, NOUT = 1, MAXT = 512, MINW = 4;
   static constexpr int PLAN = 1;
+  static constexpr bool WL = false;
   const DevParams* P;
   C2<T>* work;
   size_t nc;
@@ -609,6 +628,7 @@ k_y_pass_ch(const DevParams* __restrict__ Pp, const C2<T>* __restrict__ in, C2<T
 template <typename T> struct OpAC {
   static constexpr int NIN = 1, NOUT = 1, MAXT = 512, MINW = 4;
   static constexpr int PLAN = 0;
+  static constexpr bool WL = false;
   const DevParams* P;
   C2<T>* work;
   const C2<T>* sphi;
