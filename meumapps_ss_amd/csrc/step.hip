@@ -566,6 +566,7 @@ template <typename T, class P> struct XRows {
 template <int NC, int M, class LD, class CV, class X>
 __device__ __forceinline__ void x_c2r_pipe(LD& load, CV& c2r, X& xa, X& xb) {
   if constexpr (M + 1 < NC) load(M + 1, xb);
+  else load(-1, xb);  // the caller's own loads, if any, go out with the last component
   c2r(M, xa);
   if constexpr (M + 1 < NC) x_c2r_pipe<NC, M + 1>(load, c2r, xb, xa);
 }
@@ -591,9 +592,16 @@ template <typename T, int E> struct UntRegs {
 // buffers and component m uses buffer m mod 2; the previous user of that
 // buffer, component m - 2, finished before every thread passed the barrier
 // after component m - 1's gather, so no barrier is needed before the gather.
-template <typename T, class P, int NC, class TWS, class UN, int NBUF = 1>
+// last_loads (optional): called with no argument once the loads of the last
+// component are in flight, before it is transformed (loads the caller needs
+// after the transforms, issued early)
+struct NoLoads {
+  __device__ __forceinline__ void operator()() const {}
+};
+template <typename T, class P, int NC, class TWS, class UN, int NBUF = 1, class LL = NoLoads>
 __device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, size_t nc, int ny, C2<T>* lds0,
-                                        const TWS& twl, const UN& twn, C2<T> (&s)[NC][P::E]) {
+                                        const TWS& twl, const UN& twn, C2<T> (&s)[NC][P::E],
+                                        const LL& last_loads = LL{}) {
   constexpr int L = P::L, TPL = P::TPL, E = P::E, LP = line_pitch<L>();
   const IdxLineMajor idx{w.bl * LP};
   const size_t base = (size_t)w.z * (L + 1) * ny + (size_t)(w.yg < ny ? w.yg : ny - 1);
@@ -601,6 +609,10 @@ __device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, s
   // (component-wise copies: a whole-struct copy of double2 into the register
   // sets becomes a memcpy that keeps them in scratch memory)
   auto load = [&](int m, C2<T>(&x)[E + 1]) {
+    if (m < 0) {
+      last_loads();
+      return;
+    }
     const C2<T>* src = in + (size_t)m * nc;
 #pragma unroll
     for (int i = 0; i <= E; ++i) {
@@ -679,17 +691,17 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
 #pragma unroll
   for (int q = 0; q < E; ++q) ur.w[q] = twN[tl + TPL * q];
   C2<T> s[6][E];
-  // 1. c2r of the six strain components (in; may be the same buffer as work)
-  x_c2r_n<T, P, 6, TwRegs<T, P>, UntRegs<T, E>, 2>(w, in, nc, ny, ldsb, trw, ur, s);
-  // phi of this thread's points
+  // 1. c2r of the six strain components (in; may be the same buffer as work);
+  // phi of this thread's points is loaded while the last one is transformed
   C2<T> ph[E][V];
-  {
+  auto load_phi = [&]() {
     const size_t row = ((size_t)z * ny + (yl < ny ? yl : ny - 1)) * (size_t)(2 * L);
 #pragma unroll
     for (int q = 0; q < E; ++q)
 #pragma unroll
       for (int v = 0; v < V; ++v) ph[q][v] = ((const C2<T>*)(phi + (size_t)v * n3 + row))[tl + TPL * q];
-  }
+  };
+  x_c2r_n<T, P, 6, TwRegs<T, P>, UntRegs<T, E>, 2>(w, in, nc, ny, ldsb, trw, ur, s, load_phi);
   // 2. polarization stress at the 2E points of this thread
   if (yl < ny) {
     const double cfac = cfac_dev ? *cfac_dev : cfac_arg;
