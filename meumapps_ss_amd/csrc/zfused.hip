@@ -52,9 +52,9 @@ int launch_y_ch(const Fft3& f, const DevParams* P, const void* in, void* out, in
   switch (f.ny) {
 #define X(L_)                                                                                             \
   case L_: {                                                                                              \
-    using TL = TileLM<T, Plan<L_>>;                                                                       \
+    using TL = TileLM<T, PlanLean<L_>>;                                                                   \
     dim3 grid((nz * H + TL::B - 1) / TL::B, 1, ne);                                                       \
-    hipLaunchKernelGGL((k_y_pass_ch<T, Plan<L_>>), grid, dim3(TL::THREADS), 0, s, P, (const C2<T>*)in,    \
+    hipLaunchKernelGGL((k_y_pass_ch<T, PlanLean<L_>>), grid, dim3(TL::THREADS), 0, s, P, (const C2<T>*)in, \
                        (C2<T>*)out, f.ncplx(), (const C2<T>*)f.twNy, nz, H, e0, f.gate);                  \
     break;                                                                                                \
   }
