@@ -28,97 +28,35 @@ int launch_y_grad(const Fft3& f, const DevParams* P, const void* in, void* out, 
 
 // ============================================================== kernels
 
-// Reduction of K (<= 8) rows of nblk partials each, in two levels inside one
-// launch.  The partials were just written by the whole chip, so a single
-// workgroup reading all of them waits a memory latency per pass; instead
-// workgroup g folds rows b = g * kFinBlock + t of every slot (K loads in
-// flight per thread), each wave folds its lanes by shuffles and the waves'
-// results are folded in wave order into part2[k * G + g].  The workgroup that
-// finishes last (a counter that it resets to zero for the next launch) folds
-// the G results of each slot the same way (thread t takes g = t, t +
-// kFinBlock, ...).  The combination order is fixed, whatever the timing.
-constexpr int kFinBlock = 256, kFinMaxGroups = 256;
-__device__ __forceinline__ double rop_rt(int op, double a, double b) {
-  return op == 0 ? a + b : (op == 1 ? (b < a ? b : a) : (b > a ? b : a));
+// One workgroup of kFinBlock threads; each thread keeps four independent
+// partial results (fixed assignment of partials to accumulators), so the
+// loads of a slot are in flight together; the combination order is fixed.
+constexpr int kFinBlock = 1024;
+template <int OP> __device__ __forceinline__ double fin_slot(const double* __restrict__ p, int nblk, double* sh) {
+  const double id = OP == 0 ? 0.0 : (OP == 1 ? HUGE_VAL : -HUGE_VAL);
+  double a0 = id, a1 = id, a2 = id, a3 = id;
+  int b = threadIdx.x;
+  for (; b + 3 * kFinBlock < nblk; b += 4 * kFinBlock) {
+    const double x0 = p[b], x1 = p[b + kFinBlock], x2 = p[b + 2 * kFinBlock], x3 = p[b + 3 * kFinBlock];
+    a0 = rop<OP>(a0, x0);
+    a1 = rop<OP>(a1, x1);
+    a2 = rop<OP>(a2, x2);
+    a3 = rop<OP>(a3, x3);
+  }
+  for (; b < nblk; b += kFinBlock) a0 = rop<OP>(a0, p[b]);
+  return block_reduce<OP, kFinBlock / 64>(rop<OP>(rop<OP>(a0, a1), rop<OP>(a2, a3)), sh);
 }
-template <int K>
-__device__ __forceinline__ void fin_fold(const double* __restrict__ p, int n, const FinalizeOps& ops, double* sh,
-                                         double* r) {
-  constexpr int NW = kFinBlock / 64;
-  double a[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) a[k] = ops.op[k] == 0 ? 0.0 : (ops.op[k] == 1 ? HUGE_VAL : -HUGE_VAL);
-  for (int i = threadIdx.x; i < n; i += kFinBlock) {
-    double x[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) x[k] = p[(size_t)k * n + i];
-#pragma unroll
-    for (int k = 0; k < K; ++k) a[k] = rop_rt(ops.op[k], a[k], x[k]);
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double v = a[k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = rop_rt(ops.op[k], v, __shfl_down(v, off, 64));
-    if (lane == 0) sh[k * NW + w] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < K) {
-    const int k = threadIdx.x;
-    double v = sh[k * NW];
-#pragma unroll
-    for (int i = 1; i < NW; ++i) v = rop_rt(ops.op[k], v, sh[k * NW + i]);
-    *r = v;
-  }
-}
-template <int K>
-__global__ void __launch_bounds__(kFinBlock) k_finalize(const double* __restrict__ part, int nblk,
+__global__ void __launch_bounds__(kFinBlock) k_finalize(const double* __restrict__ part, int nblk, int K,
                                                         FinalizeOps ops, double* __restrict__ out,
-                                                        double* __restrict__ part2, unsigned* __restrict__ count,
                                                         const int* __restrict__ gate) {
   if (gated_off(gate)) return;
-  __shared__ double sh[K * (kFinBlock / 64)];
-  __shared__ int last;
-  const int G = gridDim.x, g = blockIdx.x;
-  // first level: this workgroup's rows [g * kFinBlock, min(nblk, (g + 1) * kFinBlock)) of each slot
-  {
-    constexpr int NW = kFinBlock / 64;
-    const int b = g * kFinBlock + threadIdx.x;
-    double a[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double id = ops.op[k] == 0 ? 0.0 : (ops.op[k] == 1 ? HUGE_VAL : -HUGE_VAL);
-      a[k] = b < nblk ? part[(size_t)k * nblk + b] : id;
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double v = a[k];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v = rop_rt(ops.op[k], v, __shfl_down(v, off, 64));
-      if (lane == 0) sh[k * NW + w] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < K) {
-      const int k = threadIdx.x;
-      double v = sh[k * NW];
-#pragma unroll
-      for (int i = 1; i < NW; ++i) v = rop_rt(ops.op[k], v, sh[k * NW + i]);
-      part2[k * G + g] = v;
-    }
+  __shared__ double sh[kFinBlock / 64];
+  for (int k = 0; k < K; ++k) {
+    const double* p = part + (size_t)k * nblk;
+    const int op = ops.op[k];
+    const double r = op == 0 ? fin_slot<0>(p, nblk, sh) : (op == 1 ? fin_slot<1>(p, nblk, sh) : fin_slot<2>(p, nblk, sh));
+    if (threadIdx.x == 0) out[k] = r;
   }
-  // the last workgroup to finish folds the G results of every slot
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(count, 1u) == (unsigned)(G - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double r = 0.0;
-  fin_fold<K>(part2, G, ops, sh, &r);
-  if (threadIdx.x < K) out[threadIdx.x] = r;
-  if (threadIdx.x == 0) *count = 0u;
 }
 
 // result slots in dres
@@ -1824,8 +1762,6 @@ struct mm_ctx {
   void* sstrain = nullptr;
   int nwork = 0, nxpol = 0;
   double* part = nullptr;
-  double* part2 = nullptr;      // the finalize's per-workgroup results (inside part's allocation)
-  unsigned* fin_count = nullptr;  // the finalize's workgroup counter (zero between launches)
   double* dres = nullptr;
   double* hres = nullptr;  // pinned
   int* derr = nullptr;
@@ -1895,7 +1831,6 @@ constexpr int kMaxPartSlots = 8;
 constexpr int kStageN = 6;
 constexpr int kMaxBlocks = 2048;      // grid of the pointwise and k-space kernels
 constexpr int kMaxPartBlocks = 65536; // largest grid that writes partial sums
-static_assert(kMaxPartBlocks <= kFinBlock * kFinMaxGroups, "the finalize folds every partial");
 
 int blocks_for(size_t n) { return (int)std::min<size_t>((n + kBlock - 1) / kBlock, (size_t)kMaxBlocks); }
 
@@ -1964,31 +1899,14 @@ int dalloc(void** p, size_t bytes) {
 template <typename T> T* F(void* base, size_t n, int k) { return (T*)base + (size_t)k * n; }
 template <typename T> C2<T>* S(void* base, size_t nc, int k) { return (C2<T>*)base + (size_t)k * nc; }
 
-// out slots dres[slot0 .. slot0 + K) from K rows of nblk partials
-int launch_finalize(mm_ctx* c, int nblk, int K, const FinalizeOps& fo, int slot0) {
-  double* out = c->dres + slot0;
-  const int G = (nblk + kFinBlock - 1) / kFinBlock;
-  if (nblk < 1 || G > kFinMaxGroups) return MM_ERR_ARG;
-  switch (K) {
-#define FIN_K(k) \
-  case k:                                                                                                   \
-    hipLaunchKernelGGL(k_finalize<k>, dim3(G), dim3(kFinBlock), 0, c->s, c->part, nblk, fo, out, c->part2,       \
-                       c->fin_count, c->gate);                                                                    \
-    break;
-    FIN_K(1) FIN_K(2) FIN_K(3) FIN_K(4) FIN_K(5) FIN_K(6) FIN_K(7) FIN_K(8)
-#undef FIN_K
-    default: return MM_ERR_ARG;
-  }
-  MM_HIP(hipGetLastError());
-  return 0;
-}
-
 int finalize(mm_ctx* c, int nblk, int K, std::initializer_list<int> ops, int slot0) {
   FinalizeOps fo;
   int i = 0;
   for (int o : ops) fo.op[i++] = o;
   for (; i < 8; ++i) fo.op[i] = 0;
-  return launch_finalize(c, nblk, K, fo, slot0);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, c->s, c->part, nblk, K, fo, c->dres + slot0, c->gate);
+  MM_HIP(hipGetLastError());
+  return 0;
 }
 
 void set_gate(mm_ctx* c, const int* g) {
@@ -2338,7 +2256,10 @@ int finalize_clamp(mm_ctx* c, int nblk) {
   for (int e = 0; e < E; ++e) fo.op[e] = 0;
   fo.op[E] = 1; fo.op[E + 1] = 2; fo.op[E + 2] = 2; fo.op[E + 3] = 0;
   for (int i = E + 4; i < 8; ++i) fo.op[i] = 0;
-  return launch_finalize(c, nblk, E + 4, fo, R_CLAMP);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, c->s, c->part, nblk, E + 4, fo, c->dres + R_CLAMP,
+                     c->gate);
+  MM_HIP(hipGetLastError());
+  return 0;
 }
 
 // Cahn-Hilliard update (Multi_MEUMAPPS.f:910-949) on stream s: work[4e] = c_hat,
@@ -2981,12 +2902,7 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
   rc = rc ? rc : dalloc(&c->swork, c->nwork * nc * cb);
   if (c->zfused) rc = rc ? rc : dalloc(&c->sstage, (size_t)kStageN * nc * cb);
   if (c->xpolar) rc = rc ? rc : dalloc(&c->sstrain, (size_t)6 * nc * cb);
-  rc = rc ? rc : dalloc((void**)&c->part, ((size_t)kMaxPartSlots * (kMaxPartBlocks + kFinMaxGroups) + 32) * sizeof(double));
-  if (!rc) {
-    c->part2 = c->part + (size_t)kMaxPartSlots * kMaxPartBlocks;
-    c->fin_count = (unsigned*)(c->part2 + (size_t)kMaxPartSlots * kFinMaxGroups);
-    if (hipMemset(c->fin_count, 0, 32 * sizeof(double)) != hipSuccess) rc = MM_ERR_HIP;
-  }
+  rc = rc ? rc : dalloc((void**)&c->part, (size_t)kMaxPartSlots * kMaxPartBlocks * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->dres, 64 * sizeof(double));
   rc = rc ? rc : dalloc((void**)&c->derr, 16);
   rc = rc ? rc : dalloc((void**)&c->dp, sizeof(DevParams));
