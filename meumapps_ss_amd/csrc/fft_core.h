@@ -185,6 +185,11 @@ template <> struct PlanLean4Sel<64> { using type = PlanT<64, 4, 4, 4, 4, 1>; };
 template <> struct PlanLean4Sel<128> { using type = PlanT<128, 4, 4, 4, 4, 2>; };
 template <> struct PlanLean4Sel<256> { using type = PlanT<256, 4, 4, 4, 4, 4>; };
 template <> struct PlanLean4Sel<512> { using type = PlanT<512, 4, 4, 4, 4, 4, 2>; };
+// radix-3 lengths: 6 points per thread (radix 6, then radix 2) instead of 12
+template <> struct PlanLean4Sel<12> { using type = PlanT<12, 6, 6, 2, 1, 1>; };
+template <> struct PlanLean4Sel<24> { using type = PlanT<24, 6, 6, 2, 2, 1>; };
+template <> struct PlanLean4Sel<48> { using type = PlanT<48, 6, 6, 2, 2, 2>; };
+template <> struct PlanLean4Sel<96> { using type = PlanT<96, 6, 6, 2, 2, 2, 2>; };
 template <int L> using PlanLean4 = typename PlanLean4Sel<L>::type;
 
 template <class P, int S> struct StageR {

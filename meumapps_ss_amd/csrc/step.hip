@@ -1531,8 +1531,10 @@ template <typename T> struct Kern {
 };
 
 // plan of the fused x kernel: six components of E points stay in registers
+// (binary32: the 8-point plans, and 6 points for the radix-3 lengths whose
+// lean plan holds 12)
 template <typename T, int L>
-using XPlan = std::conditional_t<sizeof(T) == 8, PlanLean4<L>, PlanLean<L>>;
+using XPlan = std::conditional_t<sizeof(T) == 8 || (PlanLean<L>::E > 8), PlanLean4<L>, PlanLean<L>>;
 template <typename T, int L> constexpr bool xplan_ok() { return XPlan<T, L>::E <= (sizeof(T) == 8 ? 4 : 8); }
 
 template <typename T, int L, int V>

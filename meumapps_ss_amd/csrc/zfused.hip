@@ -35,6 +35,10 @@ int launch_zfused(const Fft3& f, const Op& op, int ngroups, hipStream_t s, int p
 template <typename T>
 int launch_zf_elastic(const Fft3& f, const OpElastic<T>& op, int v, hipStream_t s, int probe_kind, double bytes,
                       int* nblk) {
+  // 192 and 384 have no 6-point plan within five stages; with 12 points per
+  // thread the two-waves-per-SIMD shapes spill hundreds of VGPRs, the
+  // 512-thread shape at one wave per SIMD the fewest
+  if (f.nz == 192 || f.nz == 384) v = 1;
   switch (v) {
     case 1: return launch_zfused<T>(f, OpElasticV<T, 1, 512, 1>(op), 1, s, probe_kind, bytes, nblk);
     case 2: return launch_zfused<T>(f, OpElasticV<T, 1, 256, 2, true>(op), 1, s, probe_kind, bytes, nblk);

@@ -41,6 +41,10 @@ CASES = {
     "v1_e2": (16, 1, (0.01, 0.01, 0.01), 0.005, {"numel_in": 3}, "rich"),
     # 16 x 16 x 512: the length-512 z kernels (radix-4 plan of the fused elastic pass)
     "v3_z512": (16, 3, (0.02, -0.01, -0.01), 0.005, {"nz": 512}, "rich"),
+    # 16 x 16 x 192 and x 384: radix-3 column lengths of the fused elastic pass
+    # with 12 points per thread (the 512-thread launch shape)
+    "v3_z192": (16, 3, (0.02, -0.01, -0.01), 0.005, {"nz": 192}, "rich"),
+    "v3_z384": (16, 3, (0.02, -0.01, -0.01), 0.005, {"nz": 384}, "rich"),
     # nv = 12 from sym_var rotations (InitTransGrad, ModInitials.f:200-340): an
     # orthorhombic eigenstrain turned about z in steps of 15 degrees
     "v12_rot": (16, 12, (0.02, -0.01, 0.005), 0.005, {"sym_var": "z15"}, "rich"),
