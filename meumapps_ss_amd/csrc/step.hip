@@ -2872,8 +2872,11 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
     // binary32: each column of the elastic z kernel held by whole wavefronts
     // (its LDS exchanges without workgroup barriers): 0.34 -> 0.32 ms per
     // launch at 256^3; binary64 keeps the interleaved columns (0.47 -> 0.56 ms
-    // with wavefront columns)
-    c->zf_el_var = precision == MM_MIXED ? 2 : 0;
+    // with wavefront columns).  Columns of 512: 8 points per thread so that
+    // one wavefront holds a column, at one wave per SIMD with the excess
+    // registers in AGPRs (3.63 -> 3.01 ms per launch at 512^3; at 256^3 the
+    // same shape is slower, 0.32 -> 0.37 ms)
+    c->zf_el_var = precision == MM_MIXED ? (p->nz == 512 ? 3 : 2) : 0;
     const char* zv = std::getenv("MM_ZF_ELASTIC_VARIANT");
     if (zv) c->zf_el_var = std::max(0, std::min(kZfElasticVariants - 1, std::atoi(zv)));
   }

@@ -454,13 +454,13 @@ template <typename T> struct OpElastic {
 
 // launch-shape variants of the elastic op (plan kind, workgroup size cap,
 // minimum waves per SIMD, columns held by whole wavefronts), chosen at
-// context creation by precision
+// context creation by precision and column length
 template <typename T, int PLAN_, int MAXT_, int MINW_, bool WL_ = false> struct OpElasticV : OpElastic<T> {
   static constexpr int PLAN = PLAN_, MAXT = MAXT_, MINW = MINW_;
   static constexpr bool WL = WL_;
   OpElasticV(const OpElastic<T>& b) : OpElastic<T>(b) {}
 };
-constexpr int kZfElasticVariants = 3;
+constexpr int kZfElasticVariants = 4;
 
 // ------------------------------------------------------------ Cahn-Hilliard
 // in: work[4e] = c, work[4e+1..3] = Fx, Fy, Fz after x and y; out: c_hat new.

@@ -42,6 +42,7 @@ int launch_zf_elastic(const Fft3& f, const OpElastic<T>& op, int v, hipStream_t 
   switch (v) {
     case 1: return launch_zfused<T>(f, OpElasticV<T, 1, 512, 1>(op), 1, s, probe_kind, bytes, nblk);
     case 2: return launch_zfused<T>(f, OpElasticV<T, 1, 256, 2, true>(op), 1, s, probe_kind, bytes, nblk);
+    case 3: return launch_zfused<T>(f, OpElasticV<T, 0, 256, 1, true>(op), 1, s, probe_kind, bytes, nblk);
     default: return launch_zfused<T>(f, op, 1, s, probe_kind, bytes, nblk);
   }
 }

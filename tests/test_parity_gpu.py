@@ -196,13 +196,14 @@ def test_mixed_vs_fp64(gpu_lib):
         assert abs(r["caviter"][0] - c0) < 1e-10
 
 
-@pytest.mark.parametrize("name", ["v3_inhomog", "v2_nonsq", "v3_n24", "v3_n96", "v3_z192", "v1_e2", "v12_rot",
-                                  "v2_twophase"])
+@pytest.mark.parametrize("name", ["v3_inhomog", "v2_nonsq", "v3_n24", "v3_n96", "v3_z192", "v3_z512", "v1_e2",
+                                  "v12_rot", "v2_twophase"])
 def test_mixed_cases(gpu_lib, name):
     """MIXED against FP64 on the other kernel combinations: inhomogeneous
     moduli (unfused x path), a non-cubic grid, radix-3 lengths (6 points per
     thread in the fused z kernels at 24 and 96, the 512-thread elastic shape
-    at 192), three solutes, twelve variants (unfused x path), three phases."""
+    at 192), binary32 z columns of 512 held by single wavefronts, three
+    solutes, twelve variants (unfused x path), three phases."""
     p, phi, cons = make_case(name, seed=6)
     out = {}
     for prec in (MM_FP64, MM_MIXED):
