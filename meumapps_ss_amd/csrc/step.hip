@@ -468,7 +468,12 @@ __device__ __forceinline__ void compliance_mix(const DevParams* __restrict__ P, 
 // (sum P) S otherwise, so C(x) = sc C with sc = 1 / max(1, sum P),
 // C' = (1 - sc) C, and x = C (cfac (1 - sc) b + sc a), with b = strain +
 // mean strain and a = sum_v P_v eps0_v (engineering shears).
-template <int V>
+// SYM: CEla is symmetric (checked bitwise on the host), so C(n,m) is read as
+// C(min, max): 21 distinct values instead of 36, the same products and sums.
+// In the fused x pass every thread evaluates this at 2E points with the
+// moduli, eigenstrains and mean strain as wave-uniform scalars; with 36
+// moduli the set exceeds the scalar registers and spills into vector lanes.
+template <int V, bool SYM = false>
 __device__ __forceinline__ void polar_homog(const DevParams* __restrict__ P, const double (&f)[V],
                                             const double (&e)[6], double cfac, double (&x)[6]) {
   double Pv[V], ps = 0.0;
@@ -490,7 +495,7 @@ __device__ __forceinline__ void polar_homog(const DevParams* __restrict__ P, con
   for (int m = 0; m < 6; ++m) {
     double s = 0.0;
 #pragma unroll
-    for (int n = 0; n < 6; ++n) s += P->cela[n * 6 + m] * r[n];
+    for (int n = 0; n < 6; ++n) s += P->cela[SYM ? (n < m ? n * 6 + m : m * 6 + n) : n * 6 + m] * r[n];
     x[m] = s;
   }
 }
@@ -712,14 +717,14 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
       for (int v = 0; v < V; ++v) f[v] = (double)ph[q][v].x;
 #pragma unroll
       for (int m = 0; m < 6; ++m) e[m] = (double)s[m][q].x;
-      polar_homog<V>(Pp, f, e, cfac, xo);
+      polar_homog<V, true>(Pp, f, e, cfac, xo);
 #pragma unroll
       for (int m = 0; m < 6; ++m) s[m][q].x = (T)xo[m];
 #pragma unroll
       for (int v = 0; v < V; ++v) f[v] = (double)ph[q][v].y;
 #pragma unroll
       for (int m = 0; m < 6; ++m) e[m] = (double)s[m][q].y;
-      polar_homog<V>(Pp, f, e, cfac, xo);
+      polar_homog<V, true>(Pp, f, e, cfac, xo);
 #pragma unroll
       for (int m = 0; m < 6; ++m) s[m][q].y = (T)xo[m];
     }
@@ -802,7 +807,8 @@ __device__ __forceinline__ void x_r2c_store(const XRows<T, P>& w, C2<T> (&v)[P::
 // eps = strain + w e_mean - w sum_v p_v eps0_v, every phase's energy density
 // equals the matrix one, and el_grad_v = p'(phi_v)(el_mat - el_ppt) -
 // p'(phi_v) sig:eps0_v where sum_m trans_ppt(m,v) > 1e-9.  Returns el_en.
-template <int V, int NPH>
+// SYM: CEla read as C(min, max), as in polar_homog.
+template <int V, int NPH, bool SYM = false>
 __device__ __forceinline__ double elener_homog(const DevParams* __restrict__ P, const double (&f)[V],
                                                const double (&strain)[6], double (&eg)[V]) {
   double Pv[V], ps = 0.0, te[6] = {0, 0, 0, 0, 0, 0};
@@ -824,7 +830,7 @@ __device__ __forceinline__ double elener_homog(const DevParams* __restrict__ P, 
   for (int a = 0; a < 6; ++a) {
     double s = 0.0;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) s += P->cela[c * 6 + a] * eps[c];
+    for (int c = 0; c < 6; ++c) s += P->cela[SYM ? (c < a ? c * 6 + a : a * 6 + c) : c * 6 + a] * eps[c];
     sig[a] = s;
   }
   double e2 = 0.0;
@@ -904,7 +910,7 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
         for (int v = 0; v < V; ++v) f[v] = (double)(h ? ph[v].y : ph[v].x);
 #pragma unroll
         for (int m = 0; m < 6; ++m) e[m] = (double)(h ? s[m][q].y : s[m][q].x);
-        const double een = elener_homog<V, NPH>(Pp, f, e, eg);
+        const double een = elener_homog<V, NPH, true>(Pp, f, e, eg);
         s_el += een;
         if (h) eo.y = (T)een; else eo.x = (T)een;
 #pragma unroll
@@ -2839,6 +2845,11 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
                     c->k32.xelener(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx, p->ny,
                                    p->nz, nullptr, nullptr, nullptr, nullptr, 0.0f, nullptr, nullptr, &nb);
       if (nb > kMaxPartBlocks) c->xpolar = false;
+      // the fused x kernels read CEla as a symmetric matrix (every phase has
+      // the same moduli here); a non-symmetric one runs the unfused passes
+      for (int i = 0; i < 6 && c->xpolar; ++i)
+        for (int j = i + 1; j < 6; ++j)
+          if (std::memcmp(&c->hp.cela[i * 6 + j], &c->hp.cela[j * 6 + i], sizeof(double)) != 0) c->xpolar = false;
       nb = 0;
       if (precision == MM_FP64)
         c->xclamp = c->k64.xclamp &&

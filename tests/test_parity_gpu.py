@@ -330,6 +330,36 @@ def test_mixed_state_load_restores_the_mean_composition(gpu_lib):
     assert rows_a[0]["caviter"] == rows_b[0]["caviter"]
 
 
+def test_nonsymmetric_moduli_match_oracle(gpu_lib):
+    """The fused x kernels read CEla as a symmetric matrix (21 distinct moduli,
+    checked bitwise at context creation).  Homogeneous moduli that are not
+    symmetric (C12 != C21 in every phase, compliance kept as the inverse) run
+    the unfused polarization and elener passes and must still match the
+    oracle step for step."""
+    p, phi, cons = make_case("v3_tetra", seed=3)
+    for ph in range(p.numph):
+        C = np.array([[p.cela[ph * 36 + j * 6 + i] for j in range(6)] for i in range(6)])
+        C[0, 1] += 3.0
+        S = np.linalg.inv(C)
+        for i in range(6):
+            for j in range(6):
+                p.cela[ph * 36 + j * 6 + i] = C[i, j]
+                p.sela[ph * 36 + j * 6 + i] = S[i, j]
+    nsteps = 12
+    o = Oracle(p, 4)
+    o.set_state(phi=phi, cons=cons)
+    ref_rows = o.step(nsteps)
+    with Simulation(p, MM_FP64) as s:
+        s.upload(phi=phi, cons=cons, u=np.zeros((3, *phi.shape[1:])), gradu=np.zeros((9, *phi.shape[1:])))
+        rows = s.step(nsteps)
+        st = s.download()
+    ro = o.get_state()
+    for k in ("phi", "cons", "u"):
+        err = np.abs(st[k] - ro[k]).max()
+        assert err < TOL, f"{k} max-abs {err:.3e}"
+    assert [r["n_elastic"] for r in rows] == [q["n_elastic"] for q in ref_rows]
+
+
 def test_backtracking_matches_oracle(gpu_lib):
     """Soft precipitates (0.24 x the matrix moduli) make the first elastic loop
     diverge: the error grows, cfac is cut by newfac (backtracking,
