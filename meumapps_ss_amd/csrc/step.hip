@@ -473,7 +473,11 @@ __device__ __forceinline__ void compliance_mix(const DevParams* __restrict__ P, 
 // In the fused x pass every thread evaluates this at 2E points with the
 // moduli, eigenstrains and mean strain as wave-uniform scalars; with 36
 // moduli the set exceeds the scalar registers and spills into vector lanes.
-template <int V, bool SYM = false>
+// SP: CEla has the block form of the cubic, tetragonal and hexagonal classes
+// (no coupling between normal and shear components, shear block diagonal;
+// checked exactly at context creation), so the terms that are exact zeros
+// are left out: the same sums, 12 products instead of 36.
+template <int V, bool SYM = false, bool SP = false>
 __device__ __forceinline__ void polar_homog(const DevParams* __restrict__ P, const double (&f)[V],
                                             const double (&e)[6], double cfac, double (&x)[6]) {
   double Pv[V], ps = 0.0;
@@ -495,7 +499,10 @@ __device__ __forceinline__ void polar_homog(const DevParams* __restrict__ P, con
   for (int m = 0; m < 6; ++m) {
     double s = 0.0;
 #pragma unroll
-    for (int n = 0; n < 6; ++n) s += P->cela[SYM ? (n < m ? n * 6 + m : m * 6 + n) : n * 6 + m] * r[n];
+    for (int n = 0; n < 6; ++n) {
+      if (SP && (m < 3 ? n >= 3 : n != m)) continue;
+      s += P->cela[SYM ? (n < m ? n * 6 + m : m * 6 + n) : n * 6 + m] * r[n];
+    }
     x[m] = s;
   }
 }
@@ -675,7 +682,7 @@ __device__ __forceinline__ void x_c2r_n(const XRows<T, P>& w, const C2<T>* in, s
 // the six x r2c transforms and writes the spectra back in place.  The real
 // strain and stress never reach HBM.  in and work may be the same buffer
 // (each workgroup reads its rows of every component before it writes).
-template <typename T, class P, int V>
+template <typename T, class P, int V, bool SP>
 __global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
 k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<T>* in, C2<T>* work, size_t nc,
           size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T scale, double cfac_arg,
@@ -717,14 +724,14 @@ k_x_polar(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<
       for (int v = 0; v < V; ++v) f[v] = (double)ph[q][v].x;
 #pragma unroll
       for (int m = 0; m < 6; ++m) e[m] = (double)s[m][q].x;
-      polar_homog<V, true>(Pp, f, e, cfac, xo);
+      polar_homog<V, true, SP>(Pp, f, e, cfac, xo);
 #pragma unroll
       for (int m = 0; m < 6; ++m) s[m][q].x = (T)xo[m];
 #pragma unroll
       for (int v = 0; v < V; ++v) f[v] = (double)ph[q][v].y;
 #pragma unroll
       for (int m = 0; m < 6; ++m) e[m] = (double)s[m][q].y;
-      polar_homog<V, true>(Pp, f, e, cfac, xo);
+      polar_homog<V, true, SP>(Pp, f, e, cfac, xo);
 #pragma unroll
       for (int m = 0; m < 6; ++m) s[m][q].y = (T)xo[m];
     }
@@ -808,7 +815,7 @@ __device__ __forceinline__ void x_r2c_store(const XRows<T, P>& w, C2<T> (&v)[P::
 // equals the matrix one, and el_grad_v = p'(phi_v)(el_mat - el_ppt) -
 // p'(phi_v) sig:eps0_v where sum_m trans_ppt(m,v) > 1e-9.  Returns el_en.
 // SYM: CEla read as C(min, max), as in polar_homog.
-template <int V, int NPH, bool SYM = false>
+template <int V, int NPH, bool SYM = false, bool SP = false>
 __device__ __forceinline__ double elener_homog(const DevParams* __restrict__ P, const double (&f)[V],
                                                const double (&strain)[6], double (&eg)[V]) {
   double Pv[V], ps = 0.0, te[6] = {0, 0, 0, 0, 0, 0};
@@ -830,7 +837,10 @@ __device__ __forceinline__ double elener_homog(const DevParams* __restrict__ P, 
   for (int a = 0; a < 6; ++a) {
     double s = 0.0;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) s += P->cela[SYM ? (c < a ? c * 6 + a : a * 6 + c) : c * 6 + a] * eps[c];
+    for (int c = 0; c < 6; ++c) {
+      if (SP && (a < 3 ? c >= 3 : c != a)) continue;
+      s += P->cela[SYM ? (c < a ? c * 6 + a : a * 6 + c) : c * 6 + a] * eps[c];
+    }
     sig[a] = s;
   }
   double e2 = 0.0;
@@ -866,7 +876,7 @@ __device__ __forceinline__ double elener_homog(const DevParams* __restrict__ P, 
 // XS: f_diff + el_grad is not stored as a real field but run through the x r2c
 // transform (scaled by `scale`) into the half-spectra sfd + v nc, the input of
 // the Allen-Cahn update's y pass; fdiff is then only read.
-template <typename T, class P, int V, int NPH, bool XS>
+template <typename T, class P, int V, int NPH, bool XS, bool SP>
 __global__ void __launch_bounds__((TileLM<T, P>::THREADS), 2)
 k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2<T>* __restrict__ sstrain, size_t nc,
            size_t n3, const C2<T>* __restrict__ twL, const C2<T>* __restrict__ twN, int ny, T* __restrict__ el_en,
@@ -910,7 +920,7 @@ k_x_elener(const DevParams* __restrict__ Pp, const T* __restrict__ phi, const C2
         for (int v = 0; v < V; ++v) f[v] = (double)(h ? ph[v].y : ph[v].x);
 #pragma unroll
         for (int m = 0; m < 6; ++m) e[m] = (double)(h ? s[m][q].y : s[m][q].x);
-        const double een = elener_homog<V, NPH, true>(Pp, f, e, eg);
+        const double een = elener_homog<V, NPH, true, SP>(Pp, f, e, eg);
         s_el += een;
         if (h) eo.y = (T)een; else eo.x = (T)een;
 #pragma unroll
@@ -1523,11 +1533,12 @@ template <typename T> struct Kern {
   // fused x pass + polarization stress (homogeneous moduli, supported x lengths only):
   // returns false when it has no kernel for this x length
   bool (*xpolar)(bool launch, hipStream_t, const DevParams*, const T*, const C2<T>*, C2<T>*, size_t, size_t,
-                 const void*, const void*, int, int, int, T, double, const double*, const int*) = nullptr;
+                 const void*, const void*, int, int, int, T, double, const double*, const int*, bool) = nullptr;
   // fused x c2r of the strain + elener (homogeneous moduli): nblk gets the workgroup count
   // (sfd non-null: f_diff goes out as x half-spectra into sfd, scaled by scale)
+  // the last argument of both: moduli of the block form (see polar_homog)
   bool (*xelener)(bool launch, hipStream_t, const DevParams*, const T*, const C2<T>*, size_t, size_t, const void*,
-                  const void*, int, int, int, T*, T*, T*, C2<T>*, T, double*, const int*, int*) = nullptr;
+                  const void*, int, int, int, T*, T*, T*, C2<T>*, T, double*, const int*, int*, bool) = nullptr;
   // Allen-Cahn inverse x pass + clamp + check-loop sums: nblk gets the workgroup count
   bool (*xclamp)(bool launch, hipStream_t, const DevParams*, const C2<T>*, size_t, T*, const T*, size_t, const void*,
                  const void*, int, int, int, double*, const int*, int*) = nullptr;
@@ -1546,7 +1557,7 @@ template <typename T, int L> constexpr bool xplan_ok() { return XPlan<T, L>::E <
 template <typename T, int L, int V>
 static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* in, C2<T>* work,
                       size_t nc, size_t n3, const void* twL, const void* twN, int ny, int nz, T scale, double cfac,
-                      const double* cfd, const int* gate) {
+                      const double* cfd, const int* gate, bool sp) {
   if constexpr (!xplan_ok<T, L>()) {
     return false;
   } else {
@@ -1554,8 +1565,12 @@ static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* p
     using TL = TileLM<T, XP>;
     if (launch) {
       dim3 grid((ny + TL::B - 1) / TL::B, nz);
-      hipLaunchKernelGGL((k_x_polar<T, XP, V>), grid, dim3(TL::THREADS), 0, s, P, phi, in, work, nc, n3,
-                         (const C2<T>*)twL, (const C2<T>*)twN, ny, scale, cfac, cfd, gate);
+      if (sp)
+        hipLaunchKernelGGL((k_x_polar<T, XP, V, true>), grid, dim3(TL::THREADS), 0, s, P, phi, in, work, nc, n3,
+                           (const C2<T>*)twL, (const C2<T>*)twN, ny, scale, cfac, cfd, gate);
+      else
+        hipLaunchKernelGGL((k_x_polar<T, XP, V, false>), grid, dim3(TL::THREADS), 0, s, P, phi, in, work, nc, n3,
+                           (const C2<T>*)twL, (const C2<T>*)twN, ny, scale, cfac, cfd, gate);
     }
     return true;
   }
@@ -1564,7 +1579,7 @@ static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* p
 template <typename T, int L, int V, int NPH>
 static bool go_xelener(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* sstrain, size_t nc,
                        size_t n3, const void* twL, const void* twN, int ny, int nz, T* el_en, T* elgrad, T* fdiff,
-                       C2<T>* sfd, T scale, double* part, const int* gate, int* nblk) {
+                       C2<T>* sfd, T scale, double* part, const int* gate, int* nblk, bool sp) {
   if constexpr (!xplan_ok<T, L>()) {
     return false;
   } else {
@@ -1572,16 +1587,21 @@ static bool go_xelener(bool launch, hipStream_t s, const DevParams* P, const T* 
     using TL = TileLM<T, XP>;
     dim3 grid((ny + TL::B - 1) / TL::B, nz);
     if (nblk) *nblk = (int)(grid.x * grid.y);
+#define MM_XEL(XS_, SP_)                                                                                         \
+  hipLaunchKernelGGL((k_x_elener<T, XP, V, NPH, XS_, SP_>), grid, dim3(TL::THREADS), 0, s, P, phi, sstrain, nc, n3, \
+                     (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, sfd, scale, part, gate)
     if constexpr (sizeof(T) == 8) {
       if (launch && sfd) {
-        hipLaunchKernelGGL((k_x_elener<T, XP, V, NPH, true>), grid, dim3(TL::THREADS), 0, s, P, phi, sstrain, nc,
-                           n3, (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, sfd, scale, part, gate);
+        if (sp) MM_XEL(true, true);
+        else MM_XEL(true, false);
         return true;
       }
     }
-    if (launch)
-      hipLaunchKernelGGL((k_x_elener<T, XP, V, NPH, false>), grid, dim3(TL::THREADS), 0, s, P, phi, sstrain, nc, n3,
-                         (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, sfd, scale, part, gate);
+    if (launch) {
+      if (sp) MM_XEL(false, true);
+      else MM_XEL(false, false);
+    }
+#undef MM_XEL
     return true;
   }
 }
@@ -1654,10 +1674,10 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
   }
   static bool xpolar(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* in, C2<T>* work,
                      size_t nc, size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, T scale,
-                     double cfac, const double* cfd, const int* gate) {
+                     double cfac, const double* cfd, const int* gate, bool sp) {
     switch (nx / 2) {
 #define X(L_) \
-  case L_: return go_xpolar<T, L_, V>(launch, s, P, phi, in, work, nc, n3, twL, twN, ny, nz, scale, cfac, cfd, gate);
+  case L_: return go_xpolar<T, L_, V>(launch, s, P, phi, in, work, nc, n3, twL, twN, ny, nz, scale, cfac, cfd, gate, sp);
       X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 #undef X
       default: return false;
@@ -1665,12 +1685,12 @@ template <typename T, int V, int NPH, int E> struct KernImpl {
   }
   static bool xelener(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* sstrain, size_t nc,
                       size_t n3, const void* twL, const void* twN, int nx, int ny, int nz, T* el_en, T* elgrad,
-                      T* fdiff, C2<T>* sfd, T scale, double* part, const int* gate, int* nblk) {
+                      T* fdiff, C2<T>* sfd, T scale, double* part, const int* gate, int* nblk, bool sp) {
     switch (nx / 2) {
 #define X(L_)                                                                                                  \
   case L_:                                                                                                     \
     return go_xelener<T, L_, V, NPH>(launch, s, P, phi, sstrain, nc, n3, twL, twN, ny, nz, el_en, elgrad, fdiff, \
-                                     sfd, scale, part, gate, nblk);
+                                     sfd, scale, part, gate, nblk, sp);
       X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 #undef X
       default: return false;
@@ -1785,6 +1805,7 @@ struct mm_ctx {
   // stress (needs zfused, homogeneous moduli and a fused kernel for Nx;
   // MM_XFUSED=0 switches it off)
   bool xpolar = false;
+  bool csparse = false;  // CEla of the block form: the fused kernels skip its zero terms
   // the Allen-Cahn inverse x pass runs fused with the clamp (needs zfused and a
   // fused kernel for Nx; MM_XFUSED=0 switches it off)
   bool xclamp = false;
@@ -2034,7 +2055,7 @@ template <typename T> int elastic_iter(mm_ctx* c, double cfac, const double* cfa
     // (the inverse pass of this iteration writes the new strain back)
     C2<T>* stg = (C2<T>*)c->sstrain;
     K.xpolar(true, c->s, c->dp, (const T*)c->phi, (const C2<T>*)c->sstrain, stg, nc, n3, c->fft.twLx, c->fft.twNx,
-             c->fft.nx, c->fft.ny, c->fft.nz, scale, cfac, cfac_dev, c->gate);
+             c->fft.nx, c->fft.ny, c->fft.nz, scale, cfac, cfac_dev, c->gate, c->csparse);
     c->probe.end(MM_PROBE_X_POLAR, c->s, (double)sizeof(T) * (24.0 * nc + (double)c->V * n3));
     MM_HIP(hipGetLastError());
     void* sp[6];
@@ -2351,7 +2372,8 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
       // the strain's x c2r fused with elener (the real strain is not formed)
       K.xelener(true, c->s, c->dp, phi, (const C2<T>*)c->sstrain, nc, n3, c->fft.twLx, c->fft.twNx, c->fft.nx,
                 c->fft.ny, c->fft.nz, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff,
-                xs ? (C2<T>*)c->sstage : nullptr, (T)(1.0 / (double)(float)n3), c->part, c->gate, &nblk);
+                xs ? (C2<T>*)c->sstage : nullptr, (T)(1.0 / (double)(float)n3), c->part, c->gate, &nblk,
+                c->csparse);
     } else {
       K.elener(gp, c->s, c->dp, phi, (T*)c->strain, (T*)c->el_en, (T*)c->elgrad, (T*)c->fdiff, c->part, c->derr,
                n3, c->gate);
@@ -2835,21 +2857,27 @@ extern "C" int mm_create(const mm_params* p, int precision, mm_ctx** out) {
       if (precision == MM_FP64)
         c->xpolar = c->k64.xpolar && c->k64.xelener &&
                     c->k64.xpolar(false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx,
-                                  p->ny, p->nz, 0.0, 0.0, nullptr, nullptr) &&
+                                  p->ny, p->nz, 0.0, 0.0, nullptr, nullptr, false) &&
                     c->k64.xelener(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx, p->ny,
-                                   p->nz, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, &nb);
+                                   p->nz, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, &nb, false);
       else
         c->xpolar = c->k32.xpolar && c->k32.xelener &&
                     c->k32.xpolar(false, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx,
-                                  p->ny, p->nz, 0.0f, 0.0, nullptr, nullptr) &&
+                                  p->ny, p->nz, 0.0f, 0.0, nullptr, nullptr, false) &&
                     c->k32.xelener(false, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, p->nx, p->ny,
-                                   p->nz, nullptr, nullptr, nullptr, nullptr, 0.0f, nullptr, nullptr, &nb);
+                                   p->nz, nullptr, nullptr, nullptr, nullptr, 0.0f, nullptr, nullptr, &nb, false);
       if (nb > kMaxPartBlocks) c->xpolar = false;
       // the fused x kernels read CEla as a symmetric matrix (every phase has
       // the same moduli here); a non-symmetric one runs the unfused passes
       for (int i = 0; i < 6 && c->xpolar; ++i)
         for (int j = i + 1; j < 6; ++j)
           if (std::memcmp(&c->hp.cela[i * 6 + j], &c->hp.cela[j * 6 + i], sizeof(double)) != 0) c->xpolar = false;
+      // block form (cubic, tetragonal, hexagonal): normal-shear coupling and
+      // off-diagonal shear terms exactly zero
+      c->csparse = c->xpolar;
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+          if ((i < 3 ? j >= 3 : j != i) && c->hp.cela[i * 6 + j] != 0.0) c->csparse = false;
       nb = 0;
       if (precision == MM_FP64)
         c->xclamp = c->k64.xclamp &&

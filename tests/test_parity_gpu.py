@@ -330,16 +330,22 @@ def test_mixed_state_load_restores_the_mean_composition(gpu_lib):
     assert rows_a[0]["caviter"] == rows_b[0]["caviter"]
 
 
-def test_nonsymmetric_moduli_match_oracle(gpu_lib):
-    """The fused x kernels read CEla as a symmetric matrix (21 distinct moduli,
-    checked bitwise at context creation).  Homogeneous moduli that are not
-    symmetric (C12 != C21 in every phase, compliance kept as the inverse) run
-    the unfused polarization and elener passes and must still match the
-    oracle step for step."""
+@pytest.mark.parametrize("form", ["nonsymmetric", "coupled"])
+def test_homogeneous_moduli_forms_match_oracle(gpu_lib, form):
+    """The fused x kernels read CEla as a symmetric matrix (checked bitwise at
+    context creation) and, for the block form of the cubic, tetragonal and
+    hexagonal classes, skip its zero terms.  Two homogeneous moduli outside
+    that form, in every phase with the compliance kept as the inverse:
+    nonsymmetric (C12 != C21) runs the unfused polarization and elener passes,
+    coupled (symmetric, C14 = C41 != 0) the fused kernels on the full matrix.
+    Both must match the oracle step for step."""
     p, phi, cons = make_case("v3_tetra", seed=3)
     for ph in range(p.numph):
         C = np.array([[p.cela[ph * 36 + j * 6 + i] for j in range(6)] for i in range(6)])
-        C[0, 1] += 3.0
+        if form == "nonsymmetric":
+            C[0, 1] += 3.0
+        else:
+            C[0, 3] = C[3, 0] = 5.0
         S = np.linalg.inv(C)
         for i in range(6):
             for j in range(6):
