@@ -1576,6 +1576,9 @@ static bool go_xpolar(bool launch, hipStream_t s, const DevParams* P, const T* p
   }
 }
 
+// x lengths (Nx/2) whose binary32 elener hand-over runs on the 4-point plan
+constexpr bool xs32_rows(int L) { return L == 8 || L == 16 || L == 32 || L == 64 || L == 128 || L == 256; }
+
 template <typename T, int L, int V, int NPH>
 static bool go_xelener(bool launch, hipStream_t s, const DevParams* P, const T* phi, const C2<T>* sstrain, size_t nc,
                        size_t n3, const void* twL, const void* twN, int ny, int nz, T* el_en, T* elgrad, T* fdiff,
@@ -1594,6 +1597,21 @@ static bool go_xelener(bool launch, hipStream_t s, const DevParams* P, const T* 
       if (launch && sfd) {
         if (sp) MM_XEL(true, true);
         else MM_XEL(true, false);
+        return true;
+      }
+    } else if constexpr (PlanLean4<L>::E == 4) {
+      // binary32 hand-over: the 4-point plan (its own row tiling and grid)
+      if (launch && sfd) {
+        using XP4 = PlanLean4<L>;
+        using TL4 = TileLM<T, XP4>;
+        dim3 g4((ny + TL4::B - 1) / TL4::B, nz);
+        if (nblk) *nblk = (int)(g4.x * g4.y);
+#define MM_XEL4(SP_)                                                                                           \
+  hipLaunchKernelGGL((k_x_elener<T, XP4, V, NPH, true, SP_>), g4, dim3(TL4::THREADS), 0, s, P, phi, sstrain, nc, n3, \
+                     (const C2<T>*)twL, (const C2<T>*)twN, ny, el_en, elgrad, fdiff, sfd, scale, part, gate)
+        if (sp) MM_XEL4(true);
+        else MM_XEL4(false);
+#undef MM_XEL4
         return true;
       }
     }
@@ -2357,12 +2375,12 @@ template <typename T> int step_tail(mm_ctx* c, bool elastic, double* rec_fd = nu
   // hands f_diff to the Allen-Cahn update as x half-spectra in the stage (the
   // real f_diff + el_grad is not formed; fdiff keeps the chemistry's part and
   // mm_field adds el_grad on demand).  Noise steps and recorded steps need the
-  // real f_diff.  Binary64 rows of up to 256 points only: the binary32 kernel
-  // (8 points per thread) spills with the three extra transforms and runs
-  // slower than the separate x pass (0.32 -> 0.55 ms at 256^3), and so does
-  // the binary64 one on 512-point rows (5.4 -> 7.0 ms at 512^3).
-  const bool xs = sizeof(T) == 8 && c->fft.nx <= 256 && elastic && c->xpolar && c->zfused && c->ch_done &&
-                  c->step > c->prm.noise_step && !rec_fd;
+  // real f_diff.  Rows of up to 256 points only (the binary64 kernel on
+  // 512-point rows spills: 5.4 -> 7.0 ms at 512^3); binary32 runs it on the
+  // 4-point plan (8 points per thread spill with the three extra transforms)
+  // and so only for the power-of-two rows that plan covers.
+  const bool xs = (sizeof(T) == 8 || xs32_rows(c->fft.nx / 2)) && c->fft.nx <= 256 && elastic && c->xpolar &&
+                  c->zfused && c->ch_done && c->step > c->prm.noise_step && !rec_fd;
   c->xs_tail = xs;
   c->fd_split = xs;
   if (elastic) {
